@@ -1,0 +1,375 @@
+// sw_host.cpp -- host orchestration of the Smith-Waterman kernels: the
+// alignSequences drop-in (plain per-pair kernel) and the batched
+// length-binned path (device-side binning, persistent int16 kernel, int32
+// re-run of overflow-suspect pairs, walk-back).
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "sw_host.hpp"
+#include "sw_types.hpp"
+
+namespace swift {
+
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+const char *last_error() { return g_err; }
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int DevBuf::ensure(size_t want) {
+    if (want <= bytes) return kOk;
+    if (ptr) SWIFT_HIP_TRY(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    size_t b = want + want / 8 + 256;
+    SWIFT_HIP_TRY(hipMalloc(&ptr, b));
+    bytes = b;
+    return kOk;
+}
+DevBuf::~DevBuf() {
+    // buffers live until the process ends; freeing at exit races the
+    // runtime's own teardown
+}
+
+Tuning &tuning() {
+    static Tuning t;
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Per-device workspace.
+// ---------------------------------------------------------------------------
+struct Workspace {
+    // plain path
+    DevBuf p_trace, p_res, p_ptr;
+    // batched path
+    DevBuf trace, res, ptr, bin_of, order, tasks, task_words;
+    DevBuf bins;      // bin_count, bin_cursor (x2 passes)
+    DevBuf segs;      // seg_task0 / seg_first / seg_len (x2 passes)
+    DevBuf counters;  // int32 counters
+    DevBuf totals;    // int64 total trace words per pass
+    int cu_count = 0;
+    BatchStats stats{};
+};
+
+static std::mutex g_mu;
+
+static Workspace &ws() {
+    static std::vector<Workspace *> all;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if ((int)all.size() <= dev) all.resize(dev + 1, nullptr);
+    if (!all[dev]) {
+        all[dev] = new Workspace();
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) all[dev]->cu_count = prop.multiProcessorCount;
+        if (all[dev]->cu_count <= 0) all[dev]->cu_count = 256;
+    }
+    return *all[dev];
+}
+
+BatchStats last_batch_stats() { return ws().stats; }
+
+int batch_grid() {
+    Workspace &w = ws();
+    int waves = w.cu_count * tuning().waves_per_cu;
+    return (waves + 3) / 4;  // 4 waves per 256-thread block
+}
+
+// ---------------------------------------------------------------------------
+// Scoring checks.  Every key must fit the int32 lanes: scores stay below
+// 2^19 (the best-cell key packs H << 12 with the column), so the largest
+// reachable score (query length x largest positive score, plus gaps when a
+// gap "penalty" is positive) is bounded up front.
+// ---------------------------------------------------------------------------
+int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32) {
+    int pos = std::max(std::max(s.match, s.mismatch), std::max(s.gap_open, s.gap_ext));
+    pos = std::max(pos, 0);
+    int neg = std::min(std::min(s.match, s.mismatch), std::min(s.gap_open, s.gap_ext));
+    int64_t bound = (int64_t)(maxq + maxr) * pos;
+    if (bound >= kMaxScore32 || neg < -(1 << 20)) {
+        set_error("scores outside the supported range (max reachable score %lld)",
+                  (long long)bound);
+        return kErrInvalid;
+    }
+    // int16 lanes: keys 4*H + 3 within int16, penalties small enough that
+    // the lowest candidate stays above -32768
+    *force32 = pos > 2000 || neg < -2000;
+    return kOk;
+}
+
+static int to_int_score(float f, int *out, const char *what) {
+    if (!(std::fabs(f) < 1e7f) || std::floor(f) != f) {
+        set_error("%s = %g is not an integer score (the reference parses scores with atoi)",
+                  what, (double)f);
+        return kErrInvalid;
+    }
+    *out = (int)f;
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Plain path (alignSequences drop-in).
+// ---------------------------------------------------------------------------
+static size_t g_plain_budget_words = (size_t)1 << 28;  // 1 GiB of trace
+
+int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match, float mismatch,
+                           float gap_open, float gap_ext, float *score_d, int *start_d,
+                           int *end_d, int *alen_d, char *qcons_d, char *rcons_d, int npairs,
+                           hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Scoring sc;
+    int rc;
+    if ((rc = to_int_score(match, &sc.match, "match")) ||
+        (rc = to_int_score(mismatch, &sc.mismatch, "mismatch")) ||
+        (rc = to_int_score(gap_open, &sc.gap_open, "gapOpenPenalty")) ||
+        (rc = to_int_score(gap_ext, &sc.gap_ext, "gapExtPenalty")))
+        return rc;
+    if (npairs < 0 || qlen < 0 || rlen < 0) {
+        set_error("negative sizes");
+        return kErrInvalid;
+    }
+    if (npairs == 0) return kOk;
+    if (plain_rows_for(qlen) < 0 || rlen > kMaxCols - 64) {
+        set_error("query length %d / reference length %d beyond the plain kernel (<= 1024 / %d)",
+                  qlen, rlen, kMaxCols - 64);
+        return kErrCapacity;
+    }
+    bool f32;
+    if ((rc = validate_scoring(sc, qlen, rlen, &f32))) return rc;
+    Workspace &w = ws();
+    const int R = plain_rows_for(qlen);
+    const int64_t wpp = (int64_t)(rlen + 63) * plain_trace_words(R) * 64;
+    int chunk = (int)std::min<int64_t>(npairs, std::max<int64_t>(1, g_plain_budget_words / wpp));
+    if ((rc = w.p_trace.ensure((size_t)chunk * wpp * 4))) return rc;
+    if ((rc = w.p_res.ensure((size_t)chunk * sizeof(PairResult)))) return rc;
+    if ((rc = w.p_ptr.ensure((size_t)chunk * sizeof(PairTrace)))) return rc;
+    for (int c0 = 0; c0 < npairs; c0 += chunk) {
+        int cn = std::min(chunk, npairs - c0);
+        SWIFT_HIP_TRY(launch_plain(q_d, r_d, qlen, rlen, sc, cn, c0, w.p_trace.as<uint32_t>(),
+                                   wpp, w.p_res.as<PairResult>(), w.p_ptr.as<PairTrace>(), st));
+        WalkArgs wa{};
+        wa.res = w.p_res.as<PairResult>();
+        wa.ptr = w.p_ptr.as<PairTrace>();
+        wa.trace = w.p_trace.as<uint32_t>();
+        wa.qseq = q_d;
+        wa.qstride = qlen + 1;
+        wa.rseq = r_d;
+        wa.rstride = rlen + 1;
+        wa.pair0 = c0;
+        wa.npairs = cn;
+        wa.res_global = 0;
+        wa.score_f = score_d;
+        wa.start = start_d;
+        wa.end = end_d;
+        wa.alen = alen_d;
+        wa.qcons = qcons_d;
+        wa.rcons = rcons_d;
+        wa.cstride = qlen + rlen + 2;
+        SWIFT_HIP_TRY(launch_walk(wa, st));
+    }
+    return kOk;
+}
+
+void set_plain_budget_words(size_t words) { g_plain_budget_words = std::max<size_t>(words, 1); }
+
+// ---------------------------------------------------------------------------
+// Batched path.
+// ---------------------------------------------------------------------------
+static int max_tasks_for(int npairs) { return (npairs + kSlots32 - 1) / kSlots32 + 2 * kRBins + 1; }
+
+static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
+    int R = std::max(1, (maxq + kBatchG - 1) / kBatchG);
+    int64_t steps = maxr + kBatchG - 1;
+    int64_t w16 = (R + 7) / 8, w32 = (r32_class(R) + 15) / 16;
+    int64_t t16 = (npairs + kSlots16 - 1) / kSlots16 + 2 * kRBins;
+    int64_t t32 = (npairs + kSlots32 - 1) / kSlots32 + 2 * kRBins;
+    return (t16 * w16 + 2 * t32 * w32) * steps * 64;
+}
+
+static int reserve_common(Workspace &w, int npairs) {
+    int rc;
+    int mt = max_tasks_for(npairs);
+    if ((rc = w.res.ensure((size_t)std::max(npairs, 1) * sizeof(PairResult)))) return rc;
+    if ((rc = w.ptr.ensure((size_t)std::max(npairs, 1) * sizeof(PairTrace)))) return rc;
+    if ((rc = w.bin_of.ensure((size_t)std::max(npairs, 1) * 4))) return rc;
+    if ((rc = w.order.ensure((size_t)std::max(npairs, 1) * 4))) return rc;
+    if ((rc = w.tasks.ensure((size_t)mt * sizeof(Task)))) return rc;
+    if ((rc = w.task_words.ensure((size_t)mt * 8))) return rc;
+    if ((rc = w.bins.ensure((size_t)4 * kNumBins * 4))) return rc;
+    if ((rc = w.segs.ensure((size_t)2 * 3 * (2 * kRBins + 1) * 4))) return rc;
+    if ((rc = w.counters.ensure(256 * 4))) return rc;
+    if ((rc = w.totals.ensure(8 * 8))) return rc;
+    return kOk;
+}
+
+int reserve_batch(int npairs, int maxq, int maxr) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Workspace &w = ws();
+    int rc = reserve_common(w, npairs);
+    if (rc) return rc;
+    return w.trace.ensure((size_t)worst_trace_words(npairs, maxq, maxr) * 4);
+}
+
+// counters layout (int32): [4*pass] ntasks, [4*pass+1] binned pairs, then
+// one work-queue cursor per (pass, segment) from kCntQueues on
+constexpr int kCntError = 8;
+constexpr int kCntQueues = 16;
+constexpr int kQueuesPerPass = 2 * kRBins;
+int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync_sizes) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (io.npairs < 0) {
+        set_error("negative pair count");
+        return kErrInvalid;
+    }
+    if (io.maxq > kMaxBatchQ || io.maxr > kMaxBatchR) {
+        set_error("batched kernels take queries <= %d and references <= %d (got %d, %d)",
+                  kMaxBatchQ, kMaxBatchR, io.maxq, io.maxr);
+        return kErrCapacity;
+    }
+    bool force32 = false;
+    int rc = validate_scoring(sc, io.maxq, io.maxr, &force32);
+    if (rc) return rc;
+    if (!tuning().int16) force32 = true;
+    Workspace &w = ws();
+    w.stats = BatchStats{};
+    if (io.npairs == 0) return kOk;
+    if ((rc = reserve_common(w, io.npairs))) return rc;
+    const int n = io.npairs;
+    const int mt = max_tasks_for(n);
+    int32_t *cnt = w.counters.as<int32_t>();
+    int64_t *tot = w.totals.as<int64_t>();
+    int32_t *bins = w.bins.as<int32_t>();
+    int32_t *segs = w.segs.as<int32_t>();
+    const int NS = 2 * kRBins + 1;
+
+    auto pass_args = [&](int pass, const PairResult *prev, int f32, BinArgs &b, TaskArgs &t) {
+        b = BinArgs{io.rseq, io.roff, io.qlen, io.rlen, prev, n, f32, w.bin_of.as<int32_t>(),
+                    bins + (2 * pass) * kNumBins, cnt + kCntError};
+        t = TaskArgs{bins + (2 * pass) * kNumBins, bins + (2 * pass + 1) * kNumBins,
+                     segs + (3 * pass) * NS, segs + (3 * pass + 1) * NS,
+                     segs + (3 * pass + 2) * NS, cnt + 4 * pass, cnt + 4 * pass + 1};
+    };
+    KeyConsts kc16 = make_consts(sc, kOneHot), kc32 = make_consts(sc, kBytes);
+    int pos = std::max(std::max(std::max(sc.match, sc.mismatch), std::max(sc.gap_open, sc.gap_ext)), 0);
+    uint32_t h_limit = (uint32_t)(32767 - (4 * pos + 3) - 4);
+
+    // results start at "score 0": a pair no kernel reaches never walks
+    SWIFT_HIP_TRY(launch_fill(w.res.as<int32_t>(), n * (int)(sizeof(PairResult) / 4), 0, st));
+    SWIFT_HIP_TRY(launch_fill(cnt + kCntError, 1, 0, st));
+    // ---- pass 1: bin every pair
+    BinArgs b1;
+    TaskArgs t1;
+    pass_args(0, nullptr, force32 ? 1 : 0, b1, t1);
+    SWIFT_HIP_TRY(launch_binning(b1, t1, w.order.as<int32_t>(), w.tasks.as<Task>(),
+                                 w.task_words.as<int64_t>(), nullptr, tot + 0, mt, st));
+    int64_t words1 = 0;
+    if (sync_sizes) {
+        SWIFT_HIP_TRY(hipMemcpyAsync(&words1, tot + 0, 8, hipMemcpyDeviceToHost, st));
+        SWIFT_HIP_TRY(hipStreamSynchronize(st));
+        if ((rc = w.trace.ensure((size_t)words1 * 4 + 4))) return rc;
+    }
+    SWIFT_HIP_TRY(launch_fill(cnt + kCntQueues, 2 * kQueuesPerPass, 0, st));
+    BatchArgs ba{io.qseq, io.qoff, io.qlen, io.rseq, io.roff, io.rlen, w.order.as<int32_t>(),
+                 w.tasks.as<Task>(), nullptr, nullptr, nullptr, w.trace.as<uint32_t>(),
+                 w.res.as<PairResult>(), w.ptr.as<PairTrace>(), kc16, h_limit};
+    const int grid = batch_grid();
+    // one persistent kernel per (class, R) segment, longest rows first; each
+    // reads its task range from the device, so empty segments exit at once
+    auto launch_class = [&](const TaskArgs &t, int cls, int pass) -> int {
+        for (int R = kRBins; R >= 1; --R) {
+            if (cls == 1 && r32_class(R) != R) continue;
+            int seg = cls * kRBins + (kRBins - R);
+            ba.task_lo = t.seg_task0 + seg;
+            ba.task_hi = t.seg_task0 + seg + 1;
+            ba.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
+            ba.kc = cls == 0 ? kc16 : kc32;
+            SWIFT_HIP_TRY(launch_batch(ba, cls == 0, R, grid, st));
+        }
+        return kOk;
+    };
+    if (!force32 && (rc = launch_class(t1, 0, 0))) return rc;
+    if ((rc = launch_class(t1, 1, 0))) return rc;
+
+    // ---- pass 2: int32 re-run of pairs flagged by the int16 kernel
+    int64_t words2 = words1;
+    if (!force32) {
+        BinArgs b2;
+        TaskArgs t2;
+        pass_args(1, w.res.as<PairResult>(), 1, b2, t2);
+        SWIFT_HIP_TRY(launch_binning(b2, t2, w.order.as<int32_t>(), w.tasks.as<Task>(),
+                                     w.task_words.as<int64_t>(), tot + 0, tot + 1, mt, st));
+        int32_t nre = 0;
+        if (sync_sizes) {
+            SWIFT_HIP_TRY(hipMemcpyAsync(&words2, tot + 1, 8, hipMemcpyDeviceToHost, st));
+            SWIFT_HIP_TRY(hipMemcpyAsync(&nre, cnt + 5, 4, hipMemcpyDeviceToHost, st));
+            SWIFT_HIP_TRY(hipStreamSynchronize(st));
+            w.stats.nrerun = nre;
+            if ((size_t)words2 * 4 + 4 > w.trace.bytes) {
+                // grow, keeping pass-1 traces
+                DevBuf bigger;
+                if ((rc = bigger.ensure((size_t)words2 * 4 + 4))) return rc;
+                SWIFT_HIP_TRY(hipMemcpyAsync(bigger.ptr, w.trace.ptr, (size_t)words1 * 4,
+                                             hipMemcpyDeviceToDevice, st));
+                SWIFT_HIP_TRY(hipStreamSynchronize(st));
+                std::swap(w.trace.ptr, bigger.ptr);
+                std::swap(w.trace.bytes, bigger.bytes);
+                SWIFT_HIP_TRY(hipFree(bigger.ptr));
+                bigger.ptr = nullptr;
+                ba.trace = w.trace.as<uint32_t>();
+            }
+        }
+        if (!sync_sizes || nre > 0) {
+            ba.tasks = w.tasks.as<Task>();
+            if ((rc = launch_class(t2, 1, 1))) return rc;
+        }
+    }
+    w.stats.trace_words = words2;
+
+    // ---- walk-back for every pair
+    WalkArgs wa{};
+    wa.res = w.res.as<PairResult>();
+    wa.ptr = w.ptr.as<PairTrace>();
+    wa.trace = w.trace.as<uint32_t>();
+    wa.qseq = io.qseq;
+    wa.qoff = io.qoff;
+    wa.rseq = io.rseq;
+    wa.roff = io.roff;
+    wa.qlen = io.qlen;
+    wa.rlen = io.rlen;
+    wa.pair0 = 0;
+    wa.npairs = n;
+    wa.res_global = 1;
+    wa.score_i = io.score;
+    wa.start = io.start;
+    wa.end = io.end;
+    wa.alen = io.alen;
+    wa.qcons = io.qcons;
+    wa.rcons = io.rcons;
+    wa.coff = io.cons_off;
+    SWIFT_HIP_TRY(launch_walk(wa, st));
+    if (sync_sizes) {
+        int32_t c[16];
+        SWIFT_HIP_TRY(hipMemcpyAsync(c, cnt, 16 * 4, hipMemcpyDeviceToHost, st));
+        SWIFT_HIP_TRY(hipStreamSynchronize(st));
+        int32_t segbuf[2 * kRBins + 1];
+        SWIFT_HIP_TRY(hipMemcpy(segbuf, t1.seg_task0, sizeof(segbuf), hipMemcpyDeviceToHost));
+        w.stats.ntasks16 = force32 ? 0 : segbuf[kRBins];
+        w.stats.ntasks32 = segbuf[2 * kRBins] - segbuf[kRBins];
+        if (c[kCntError]) {
+            set_error("pair lengths beyond maxq/maxr or the batched limits (%d/%d)", kMaxBatchQ,
+                      kMaxBatchR);
+            return kErrCapacity;
+        }
+    }
+    return kOk;
+}
+
+}  // namespace swift

@@ -1,0 +1,71 @@
+// sw_host.hpp -- declarations shared by the host-side translation units.
+#pragma once
+
+#include "swift_common.hpp"
+
+namespace swift {
+
+struct KeyConsts;
+struct Task;
+struct BatchArgs;
+struct WalkArgs;
+struct BinArgs;
+struct TaskArgs;
+
+// launch helpers implemented in sw_kernels.hip
+hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
+                        int npairs, int pair0, uint32_t *trace, int64_t wpp, PairResult *res,
+                        PairTrace *ptr, hipStream_t st);
+int plain_rows_for(int qlen);
+int plain_trace_words(int R);
+hipError_t launch_walk(const WalkArgs &wa, hipStream_t st);
+hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st);
+
+// Device buffer that only grows.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want);
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+    ~DevBuf();
+};
+
+// Batched SW over pairs already resident on the device.  Lengths live on
+// the device; maxq/maxr bound them (host-side validation and buffer sizing).
+// Results: int32 score/start/end/alen and reversed consensus strings at
+// cons_off[p] (capacity qlen + rlen + 2 each).  When `sync_sizes` is false
+// the call allocates nothing and relies on `reserve_batch` having sized the
+// workspace for (npairs, maxq, maxr); that form is graph-capturable.
+struct BatchIO {
+    const char *qseq;
+    const int64_t *qoff;
+    const int32_t *qlen;
+    const char *rseq;
+    const int64_t *roff;
+    const int32_t *rlen;
+    int npairs, maxq, maxr;
+    int32_t *score, *start, *end, *alen;
+    char *qcons, *rcons;
+    const int64_t *cons_off;
+};
+
+int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32);
+int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync_sizes);
+int reserve_batch(int npairs, int maxq, int maxr);
+int batch_grid();
+
+// statistics of the last batched run (for tests and the bench)
+struct BatchStats {
+    int ntasks16, ntasks32, nrerun;
+    int64_t trace_words;
+};
+BatchStats last_batch_stats();
+
+// tuning knobs set by the autotuner (see tune.cpp)
+struct Tuning {
+    int waves_per_cu = 8;       // persistent grid size
+    int int16 = 1;              // allow the int16 packed path
+};
+Tuning &tuning();
+
+}  // namespace swift

@@ -1,0 +1,134 @@
+// sw_types.hpp -- kernel argument structures and layout constants shared by
+// sw_kernels.hip and the host code.
+#pragma once
+
+#include "swift_common.hpp"
+
+namespace swift {
+
+// per-launch constants in key space (key = 4*H + direction)
+struct KeyConsts {
+    int dr;    // added to 4H to form the diagonal-ready value
+    int dsub;  // added when the substitution flag is set
+    int eo2;   // 4*gapOpen + 2 (up exit, open)
+    int eo1;   // 4*gapOpen + 1 (left exit, open)
+    int de;    // 4*(gapExt - gapOpen)
+};
+
+// ONEHOT: flag = match,    dr = 4*mismatch + 3, dsub = 4*(match - mismatch)
+// BYTES : flag = mismatch, dr = 4*match + 3,    dsub = 4*(mismatch - match)
+enum Policy { kOneHot = 0, kBytes = 1 };
+
+inline KeyConsts make_consts(const Scoring &s, int policy) {
+    KeyConsts k;
+    if (policy == kOneHot) {
+        k.dr = 4 * s.mismatch + 3;
+        k.dsub = 4 * (s.match - s.mismatch);
+    } else {
+        k.dr = 4 * s.match + 3;
+        k.dsub = 4 * (s.mismatch - s.match);
+    }
+    k.eo2 = 4 * s.gap_open + 2;
+    k.eo1 = 4 * s.gap_open + 1;
+    k.de = 4 * (s.gap_ext - s.gap_open);
+    return k;
+}
+
+constexpr int kColBits = 12;                   // int32 best key: H << 12 | (4095 - j)
+constexpr int kMaxCols = (1 << kColBits) - 1;  // longest reference window (plain kernel)
+constexpr int kMaxScore32 = 1 << 19;           // scores must stay below this
+constexpr int kBatchG = 16;                    // lanes per pair group (batched kernels)
+constexpr int kBatchCols = 512;                // reference columns per group (LDS)
+constexpr int kRBins = 20;                     // R = 1..20 rows per lane (query <= 320)
+constexpr int kLenBins = kBatchCols / 8;       // bins of 8 reference columns
+constexpr int kNumBins = 2 * kRBins * kLenBins;
+constexpr int kMaxBatchQ = kBatchG * kRBins;   // 320
+constexpr int kMaxBatchR = kBatchCols - 1;     // 511
+constexpr int kSlots16 = 8;                    // pairs per wave task, int16 packed
+constexpr int kSlots32 = 4;                    // pairs per wave task, int32
+
+__host__ __device__ inline int r32_class(int r) { return r <= 2 ? 2 : (r + 1) & ~1; }
+
+struct Task {
+    int64_t trace_base;  // word offset
+    int32_t first;       // index of the task's first pair in the sorted order
+    int16_t npairs;      // pairs in the task (<= slots per wave)
+    int16_t rows;        // R
+    int32_t steps;       // max rlen + G - 1
+    int32_t min_rlen;    // smallest rlen among the task's pairs
+};
+
+struct BatchArgs {
+    const char *qseq;
+    const int64_t *qoff;
+    const int32_t *qlen;
+    const char *rseq;
+    const int64_t *roff;
+    const int32_t *rlen;
+    const int32_t *order;     // sorted pair order
+    const Task *tasks;
+    const int32_t *task_lo;   // device: first task of this kernel's class
+    const int32_t *task_hi;   // device: one past its last task
+    int32_t *next_task;       // work-queue cursor (zeroed before launch)
+    uint32_t *trace;
+    PairResult *res;
+    PairTrace *ptr;
+    KeyConsts kc;
+    uint32_t h_limit;         // 4*H at or above this flags an int16 overflow risk
+};
+
+struct WalkArgs {
+    const PairResult *res;
+    const PairTrace *ptr;
+    const uint32_t *trace;
+    // sequences: per pair offsets, or fixed strides when the offsets are null
+    const char *qseq;
+    const int64_t *qoff;
+    int64_t qstride;
+    const char *rseq;
+    const int64_t *roff;
+    int64_t rstride;
+    const int32_t *qlen;     // per pair lengths (null: strides - 1)
+    const int32_t *rlen;
+    int pair0;               // first pair of this launch; res/ptr are indexed from it
+    int npairs;
+    int res_global;          // 1: res/ptr indexed by the global pair id
+    // outputs
+    float *score_f;          // drop-in layout (float score) or
+    int32_t *score_i;        // batched layout (int score)
+    int32_t *start, *end, *alen;
+    char *qcons, *rcons;
+    const int64_t *coff;     // per pair consensus offset, or fixed stride
+    int64_t cstride;
+};
+
+struct BinArgs {
+    const char *rseq;
+    const int64_t *roff;
+    const int32_t *qlen;
+    const int32_t *rlen;
+    const PairResult *prev;   // rerun pass: only flagged pairs take part
+    int npairs;
+    int force32;              // scoring outside the int16 range: all pairs int32
+    int32_t *bin_of;          // per pair bin (or -1 when not in this pass)
+    int32_t *bin_count;       // kNumBins
+    int32_t *error;           // set to nonzero when a pair's lengths are out of range
+};
+
+struct TaskArgs {
+    const int32_t *bin_count;
+    int32_t *bin_cursor;      // exclusive prefix of bin_count (scatter cursor)
+    int32_t *seg_task0;       // first task of each (class, R) segment (+1 sentinel)
+    int32_t *seg_first;       // first sorted pair of each segment
+    int32_t *seg_len;         // pairs in each segment
+    int32_t *ntasks;
+    int32_t *npairs_binned;
+};
+
+// launch helpers (sw_kernels.hip)
+hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int grid, hipStream_t st);
+hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order, Task *tasks,
+                          int64_t *task_words, const int64_t *base0, int64_t *total_words,
+                          int max_tasks, hipStream_t st);
+
+}  // namespace swift

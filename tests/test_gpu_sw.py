@@ -1,0 +1,171 @@
+"""Phase-2 parity on the GPU: the HIP kernels against the oracle on the
+same seeded inputs, bit for bit (scores, start/end, lengths, consensus
+strings)."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_vectors.json")))
+KEYS = ("score", "start", "end", "len", "qry_cons", "ref_cons")
+
+
+@pytest.fixture(scope="module")
+def A():
+    from swift_amd import align
+    return align
+
+
+def compare(got, orc_res, ctx):
+    for k in KEYS:
+        assert got[k] == orc_res[k], (ctx, k, got, {k2: orc_res[k2] for k2 in KEYS})
+
+
+def rand_pairs(rnd, n, qmax, rmax, alphas):
+    qs, rs = [], []
+    for _ in range(n):
+        al = rnd.choice(alphas)
+        qs.append(bytes(rnd.choice(al) for _ in range(rnd.randint(1, qmax))))
+        rs.append(bytes(rnd.choice(al) for _ in range(rnd.randint(1, rmax))))
+    return qs, rs
+
+
+def test_plain_known_answers(A, orc):
+    for case in GOLD["sw"]:
+        m, x, o, e = case["scoring"]
+        ref = case["ref"].encode()
+        rl = case.get("ref_len", len(ref))
+        res = A.align_sequences([case["qry"].encode()], [ref], ref_len=rl,
+                                scoring=A.Scoring(m, x, o, e))
+        got = res.record(0)
+        assert got["score"] == case["score"]
+        if "start" in case:
+            assert (got["start"], got["end"], got["len"]) == (case["start"], case["end"],
+                                                               case["len"])
+            assert got["qry_cons"] == case["qry_cons"].encode()
+            assert got["ref_cons"] == case["ref_cons"].encode()
+
+
+SCORINGS = [(2, -1, -10, -1), (1, -1, -1, -1), (1, 0, -1, 0), (3, -2, -2, -1),
+            (2, -1, -1, -2), (1, -1, 0, 0), (2, 1, -3, -1)]
+
+
+@pytest.mark.parametrize("sc", SCORINGS)
+def test_plain_random_vs_oracle(A, orc, sc):
+    """alignSequences drop-in: uniform strides, NUL-terminated references of
+    varying length, every query byte a row (exotic bytes included)."""
+    rnd = random.Random(hash(sc) & 0xFFFF)
+    ql, rl = 37, 90
+    alphas = [b"ACGT", b"AC", b"ACGTN", b"ACGTacgtNn", b"ACGTRY*"]
+    qs = [bytes(rnd.choice(rnd.choice(alphas)) for _ in range(ql)) for _ in range(300)]
+    rs = [bytes(rnd.choice(rnd.choice(alphas)) for _ in range(rnd.randint(1, rl)))
+          for _ in range(300)]
+    res = A.align_sequences(qs, rs, ql, rl, A.Scoring(*sc))
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i], *sc, ref_len=rl), (i, sc))
+
+
+@pytest.mark.parametrize("ql,rl", [(100, 300), (64, 300), (65, 128), (1, 50), (250, 450),
+                                   (150, 300)])
+def test_plain_shapes_vs_oracle(A, orc, ql, rl):
+    rnd = random.Random(ql * 1000 + rl)
+    qs, rs = [], []
+    for _ in range(64):
+        reg = bytes(rnd.choice(b"ACGT") for _ in range(rl))
+        o = rnd.randint(0, max(0, rl - ql))
+        q = bytearray(reg[o:o + ql].ljust(ql, b"A"))
+        for k in range(ql):
+            if rnd.random() < 0.04:
+                q[k] = rnd.choice(b"ACGT")
+        qs.append(bytes(q))
+        rs.append(reg[:rnd.randint(max(1, rl - 40), rl)])
+    res = A.align_sequences(qs, rs, ql, rl)
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i], ref_len=rl), (ql, rl, i))
+
+
+def test_plain_exit_on_bad_score(A):
+    with pytest.raises(A.__class__ if False else Exception):
+        A.align_sequences([b"ACGT"], [b"ACGT"], scoring=A.Scoring(2.5, -1, -10, -1))
+
+
+def test_batch_uneven_vs_oracle(A, orc):
+    """Batched path over every row class: query 1..320, reference 1..511."""
+    rnd = random.Random(11)
+    qs, rs = rand_pairs(rnd, 1500, 320, 511, [b"ACGT", b"ACGTN", b"AC", b"ACGTacgtn"])
+    res = A.sw_batch(qs, rs)
+    st = A.batch_stats()
+    assert st["ntasks16"] > 0
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i]), i)
+
+
+def test_batch_c3_shapes_vs_oracle(A, orc):
+    from swift_amd import synth
+    qs, rs = synth.c3_pairs(3000, seed=5)
+    res = A.sw_batch(qs, rs)
+    sc, st, en, al, qc, rc = orc.sw_batch(qs, rs, nthreads=8)
+    assert np.array_equal(res.score, sc) and np.array_equal(res.start, st)
+    assert np.array_equal(res.end, en) and np.array_equal(res.length, al)
+    assert res.qry_cons == qc and res.ref_cons == rc
+
+
+def test_batch_exotic_bytes_route_int32(A, orc):
+    rnd = random.Random(3)
+    qs, rs = rand_pairs(rnd, 400, 120, 300, [b"ACGTRY*", b"ACGTacgtNn#"])
+    res = A.sw_batch(qs, rs)
+    assert A.batch_stats()["ntasks32"] > 0
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i]), i)
+
+
+def test_batch_int16_overflow_rerun(A, orc):
+    """match = 200: long exact matches pass the int16 range, the int16 kernel
+    flags them and the int32 re-run recomputes them exactly."""
+    rnd = random.Random(8)
+    qs, rs = [], []
+    for _ in range(200):
+        L = rnd.randint(20, 250)
+        reg = bytes(rnd.choice(b"ACGT") for _ in range(L + rnd.randint(0, 100)))
+        o = rnd.randint(0, len(reg) - L)
+        qs.append(reg[o:o + L])
+        rs.append(reg)
+    sc = A.Scoring(200, -1, -10, -1)
+    res = A.sw_batch(qs, rs, sc)
+    st = A.batch_stats()
+    assert st["nrerun"] > 0
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i], 200, -1, -10, -1), i)
+
+
+@pytest.mark.parametrize("sc", SCORINGS)
+def test_batch_scorings_vs_oracle(A, orc, sc):
+    rnd = random.Random(sum(sc) + 100)
+    qs, rs = rand_pairs(rnd, 300, 200, 400, [b"ACGT", b"AC", b"ACGTN"])
+    res = A.sw_batch(qs, rs, A.Scoring(*sc))
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i], *sc), (i, sc))
+
+
+def test_batch_edge_sizes(A, orc):
+    qs = [b"A", b"ACGT" * 80, b"T" * 16, b"G" * 17, b"ACGTN", b"CCCCCCCCCC"]
+    rs = [b"A", b"ACGT" * 127 + b"ACG", b"T", b"G" * 511, b"N", b"GGGGGGGGGG"]
+    res = A.sw_batch(qs, rs)
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i]), i)
+
+
+def test_batch_repeatable(A):
+    from swift_amd import synth
+    qs, rs = synth.c3_pairs(2000, seed=9)
+    b = A.PairBatch(qs, rs)
+    b.run()
+    r1 = b.results()
+    b.run()
+    r2 = b.results()
+    assert np.array_equal(r1.score, r2.score) and r1.qry_cons == r2.qry_cons
