@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the phase-2 hot path on one MI355X.
+
+Workload (config 3 of the benchmark plan): 1,000,000 query/region pairs,
+query length U{36..250}, region = query + U{20..200} bases with the query
+embedded after 2 % divergence, reference default scoring.  One step = one
+swift_sw_batch pass over all pairs resident in HBM (device binning, packed
+int16 forward sweep with int32 re-run of overflow suspects, traceback
+walk), i.e. scores, coordinates and consensus strings for every pair.
+
+Also measured on the same pairs: the plain per-pair drop-in
+(alignSequences, one call per query length) and the reference CPU
+smithWaterman_cpu on a bounded sample.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SW GCUPS + reads/s at 1 MI355X, batched vs per-pair; p50/p99 latency"
+CELL_OPS = 16                 # algorithmic integer ops per DP cell
+PEAK_INT16_TOPS = 78.6        # 256 CU x 64 lanes x 2 (packed) x 2.4 GHz
+PEAK_INT32_TOPS = 39.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(qbuf, qoff, qlen, rbuf, roff, rlen, npairs):
+    """The reference's own smithWaterman_cpu (oracle/_ref, compiled from the
+    reference sources) on the first `npairs` pairs, or the oracle port when
+    that library is absent."""
+    from oracle import oracle as O
+    n = min(npairs, len(qlen))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    cells = float((qlen[:n].astype(np.int64) * rlen[:n]).sum())
+    R = O.ref()
+    if R is not None:
+        qo = qoff[:n].astype(np.int64).copy()
+        ro = roff[:n].astype(np.int64).copy()
+        ql = qlen[:n].astype(np.int32).copy()
+        rl = rlen[:n].astype(np.int32).copy()
+        sc = np.zeros(n, np.int32)
+        p = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        t0 = time.perf_counter()
+        R.ref_sw_batch_mt(n, p(qbuf, C.c_char), p(qo, C.c_long), p(ql, C.c_int),
+                          p(rbuf, C.c_char), p(ro, C.c_long), p(rl, C.c_int), C.c_float(2),
+                          C.c_float(-1), C.c_float(-10), C.c_float(-1), p(sc, C.c_int), threads)
+        dt = time.perf_counter() - t0
+        kind = "reference"
+    else:
+        qs = [qbuf[qoff[i]:qoff[i] + qlen[i]].tobytes() for i in range(n)]
+        rs = [rbuf[roff[i]:roff[i] + rlen[i]].tobytes() for i in range(n)]
+        t0 = time.perf_counter()
+        O.sw_batch(qs, rs, nthreads=threads)
+        dt = time.perf_counter() - t0
+        kind = "port"
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": kind,
+            "sample": f"first {n} pairs of the workload ({cells:.3e} cells), "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
+def plain_pass_setup(qbuf, qoff, qlen, rbuf, roff, rlen):
+    """Reference-layout buffers for the plain drop-in: one alignSequences
+    call per query length, stride qlen+1 / rmax+1 (NUL padded)."""
+    from swift_amd.device import DeviceBuffer
+    groups = []
+    for q in np.unique(qlen):
+        idx = np.nonzero(qlen == q)[0]
+        q = int(q)
+        rmax = int(rlen[idx].max())
+        n = len(idx)
+        qa = np.zeros((n, q + 1), np.uint8)
+        ra = np.zeros((n, rmax + 1), np.uint8)
+        qa[:, :q] = qbuf[(qoff[idx][:, None] + np.arange(q)[None, :])]
+        cols = np.arange(rmax)[None, :]
+        mask = cols < rlen[idx][:, None]
+        src = np.minimum(roff[idx][:, None] + cols, len(rbuf) - 1)
+        ra[:, :rmax] = np.where(mask, rbuf[src], 0)
+        cl = q + rmax + 2
+        bufs = dict(q=DeviceBuffer.from_array(qa), r=DeviceBuffer.from_array(ra),
+                    sc=DeviceBuffer(4 * n), st=DeviceBuffer(4 * n), en=DeviceBuffer(4 * n),
+                    al=DeviceBuffer(4 * n), qc=DeviceBuffer(n * cl), rc=DeviceBuffer(n * cl))
+        groups.append((q, rmax, n, bufs))
+    return groups
+
+
+def plain_pass(groups):
+    from swift_amd._lib import check, lib
+    for q, rmax, n, b in groups:
+        check(lib().swift_align_sequences(
+            C.c_void_p(b["q"].ptr), C.c_void_p(b["r"].ptr), q, rmax, C.c_float(2), C.c_float(-1),
+            C.c_float(-10), C.c_float(-1), C.c_void_p(b["sc"].ptr), C.c_void_p(b["st"].ptr),
+            C.c_void_p(b["en"].ptr), C.c_void_p(b["al"].ptr), C.c_void_p(b["qc"].ptr),
+            C.c_void_p(b["rc"].ptr), n, None))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-pairs", type=int, default=800000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the alignment "
+              "hot path has no cross-GPU exchange; see DESIGN.md for how it shards.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from swift_amd import align, synth
+    from swift_amd._lib import lib
+    from swift_amd.device import synchronize
+
+    t0 = time.perf_counter()
+    qbuf, qoff, qlen, rbuf, roff, rlen = synth.c3_pairs_fast(args.pairs)
+    cells = int((qlen.astype(np.int64) * rlen).sum())
+    log(f"[bench] generated {args.pairs} pairs, {cells:.3e} cells in "
+        f"{time.perf_counter() - t0:.1f} s")
+
+    batch = align.PairBatch.from_arrays(qbuf, qoff, qlen, rbuf, roff, rlen)
+    ms4 = (C.c_float * 4)()
+    lib().swift_sw_batch_timing(1, ms4)
+    for i in range(args.warmup):
+        batch.run()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    synchronize()
+    fwd, walk, binning, rerun = [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batch.run()
+        lib().swift_sw_batch_timing(1, ms4)
+        binning.append(ms4[0])
+        fwd.append(ms4[1])
+        rerun.append(ms4[2])
+        walk.append(ms4[3])
+        log(f"[bench] step {i + 1}/{args.steps}: forward {ms4[1]:.2f} ms, walk {ms4[3]:.2f} ms")
+    synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt * 1e3 / args.steps
+    gcups = cells / (dt / args.steps) / 1e9
+    stats = align.batch_stats()
+    fwd_ms = float(np.mean(fwd))
+    achieved = CELL_OPS * cells / (fwd_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16",
+        "data": "synthetic (seeded generator, swift_amd/synth.py)",
+        "config": {"workload": "C3 batched uneven pairs: 1M query/region pairs, query "
+                               "U{36..250} bp, region = query + U{20..200} bp, 2% divergence; "
+                               "scores, coordinates and consensus strings for every pair",
+                   "pairs": args.pairs, "cells": cells, "scoring": [2, -1, -10, -1],
+                   "lanes": "packed int16 with int32 re-run"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
+                     "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
+                     "traffic": None, "kernel": "sw_batch_kernel<R,int16> (all row classes)",
+                     "ops_per_cell": CELL_OPS, "forward_ms": round(fwd_ms, 3)},
+        "phases_ms": {"binning": round(float(np.mean(binning)), 3),
+                      "forward": round(fwd_ms, 3), "int32_rerun": round(float(np.mean(rerun)), 3),
+                      "walk": round(float(np.mean(walk)), 3)},
+        "batch_stats": stats,
+    }
+    if not args.no_plain:
+        groups = plain_pass_setup(qbuf, qoff, qlen, rbuf, roff, rlen)
+        plain_pass(groups)
+        synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.plain_steps):
+            plain_pass(groups)
+            synchronize()
+            log(f"[bench] plain pass {i + 1}/{args.plain_steps} done")
+        pdt = (time.perf_counter() - t0) / args.plain_steps
+        out["plain_per_pair"] = {"gcups": round(cells / pdt / 1e9, 2),
+                                 "ms_per_pass": round(pdt * 1e3, 3),
+                                 "calls": len(groups)}
+        out["speedup_vs_plain"] = round(gcups / (cells / pdt / 1e9), 3)
+        del groups
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(qbuf, qoff, qlen, rbuf, roff, rlen, args.cpu_pairs)
+        log(f"[bench] cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -87,7 +87,7 @@ int swift_align_sequences(char *querySeq_d, char *refSeq_d, int querySeqLength, 
  * only).  npairs query/reference pairs of uneven lengths, all on the device:
  * pair p's query is qseq_d[qoff_d[p] .. +qlen_d[p]), its reference
  * rseq_d[roff_d[p] .. +rlen_d[p]).  maxq/maxr bound the lengths
- * (maxq <= 320, maxr <= 511).  Scores follow smithWaterman_cpu exactly.
+ * (maxq <= 320, maxr <= 480).  Scores follow smithWaterman_cpu exactly.
  * Outputs: score/start/end/alen per pair and the consensus strings in walk
  * order at cons_off_d[p] (room for qlen+rlen+2 bytes each, NUL terminated).
  * Pairs are length-binned on the device, run two per lane in packed int16
@@ -103,6 +103,12 @@ int swift_sw_batch(const char *qseq_d, const int64_t *qoff_d, const int32_t *qle
 /* Counters of the last swift_sw_batch call: wave tasks run with int16 and
  * int32 lanes, pairs re-run in int32, trace words used. */
 int swift_sw_batch_stats(int *ntasks16, int *ntasks32, int *nrerun, int64_t *trace_words);
+
+/* Phase timing of swift_sw_batch measured with HIP events on its stream:
+ * enable != 0 turns recording on for later calls; ms4 receives the last
+ * call's binning, forward sweep (all row classes, int16 and int32), int32
+ * re-run pass and walk-back times in ms. */
+int swift_sw_batch_timing(int enable, float *ms4);
 
 /* Tuning knobs (set by the autotuner): persistent waves per CU and whether
  * the int16 path may be used. */

@@ -91,9 +91,19 @@ class PairBatch:
     """Pairs resident on the device in the batched layout."""
 
     def __init__(self, queries, refs):
-        self.n = len(queries)
         qb, qo, qlens = pack(queries)
         rb, ro, rlens = pack(refs)
+        self._setup(qb, qo, qlens, rb, ro, rlens)
+
+    @classmethod
+    def from_arrays(cls, qbuf, qoff, qlen, rbuf, roff, rlen):
+        self = cls.__new__(cls)
+        self._setup(qbuf, np.asarray(qoff, np.int64), np.asarray(qlen, np.int32), rbuf,
+                    np.asarray(roff, np.int64), np.asarray(rlen, np.int32))
+        return self
+
+    def _setup(self, qb, qo, qlens, rb, ro, rlens):
+        self.n = len(qlens)
         self.qlen_h, self.rlen_h = qlens, rlens
         self.maxq = int(qlens.max()) if self.n else 0
         self.maxr = int(rlens.max()) if self.n else 0

@@ -98,8 +98,15 @@ int swift_sw_batch_stats(int *ntasks16, int *ntasks32, int *nrerun, int64_t *tra
     return kOk;
 }
 
+int swift_sw_batch_timing(int enable, float *ms4) {
+    set_batch_timing(enable != 0);
+    if (ms4)
+        for (int i = 0; i < 4; ++i) ms4[i] = batch_timing_ms()[i];
+    return kOk;
+}
+
 int swift_set_tuning(int waves_per_cu, int allow_int16) {
-    if (waves_per_cu < 1 || waves_per_cu > 64) {
+    if (waves_per_cu < 0 || waves_per_cu > 64) {
         set_error("waves_per_cu out of range");
         return kErrInvalid;
     }

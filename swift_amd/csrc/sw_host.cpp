@@ -46,17 +46,41 @@ Tuning &tuning() {
 // ---------------------------------------------------------------------------
 // Per-device workspace.
 // ---------------------------------------------------------------------------
+struct Timing {
+    bool on = false;
+    hipEvent_t ev[6] = {};
+    float ms[5] = {};
+};
+static Timing g_timing;
+
+static hipError_t mark(int i, hipStream_t st) {
+    if (!g_timing.on) return hipSuccess;
+    if (!g_timing.ev[i]) {
+        hipError_t e = hipEventCreate(&g_timing.ev[i]);
+        if (e != hipSuccess) return e;
+    }
+    return hipEventRecord(g_timing.ev[i], st);
+}
+
+void set_batch_timing(bool on) { g_timing.on = on; }
+const float *batch_timing_ms() { return g_timing.ms; }
+
 struct Workspace {
     // plain path
     DevBuf p_trace, p_res, p_ptr;
     // batched path
-    DevBuf trace, res, ptr, bin_of, order, tasks, task_words;
+    DevBuf trace, res, ptr, bin_of, order, order2, tasks, task_words;
     DevBuf bins;      // bin_count, bin_cursor (x2 passes)
     DevBuf segs;      // seg_task0 / seg_first / seg_len (x2 passes)
     DevBuf counters;  // int32 counters
     DevBuf totals;    // int64 total trace words per pass
     int cu_count = 0;
     BatchStats stats{};
+    // side streams the row-class kernels are spread over (fork/join by events)
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork = nullptr, join[kSide] = {};
+    bool streams_ready = false;
 };
 
 static std::mutex g_mu;
@@ -77,10 +101,15 @@ static Workspace &ws() {
 
 BatchStats last_batch_stats() { return ws().stats; }
 
-int batch_grid() {
-    Workspace &w = ws();
-    int waves = w.cu_count * tuning().waves_per_cu;
-    return (waves + 3) / 4;  // 4 waves per 256-thread block
+static int ensure_streams(Workspace &w) {
+    if (w.streams_ready) return kOk;
+    for (int i = 0; i < Workspace::kSide; ++i) {
+        SWIFT_HIP_TRY(hipStreamCreateWithFlags(&w.side[i], hipStreamNonBlocking));
+        SWIFT_HIP_TRY(hipEventCreateWithFlags(&w.join[i], hipEventDisableTiming));
+    }
+    SWIFT_HIP_TRY(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
+    w.streams_ready = true;
+    return kOk;
 }
 
 // ---------------------------------------------------------------------------
@@ -101,7 +130,10 @@ int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32) {
     }
     // int16 lanes: keys 4*H + 3 within int16, penalties small enough that
     // the lowest candidate stays above -32768
-    *force32 = pos > 2000 || neg < -2000;
+    // the packed kernel keeps the gap exits (4*gap + 1 or + 2) in signed
+    // byte tables: gaps must lie in [-31, 31]
+    bool gaps8 = s.gap_open >= -31 && s.gap_open <= 31 && s.gap_ext >= -31 && s.gap_ext <= 31;
+    *force32 = pos > 2000 || neg < -2000 || !gaps8;
     return kOk;
 }
 
@@ -146,34 +178,15 @@ int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match
     if ((rc = validate_scoring(sc, qlen, rlen, &f32))) return rc;
     Workspace &w = ws();
     const int R = plain_rows_for(qlen);
-    const int64_t wpp = (int64_t)(rlen + 63) * plain_trace_words(R) * 64;
+    // trace words per pair: whole 4-step granules of (rlen + 63) steps
+    const int64_t wpp = (int64_t)((rlen + 63 + 3) & ~3) * plain_trace_words(R) * 64;
     int chunk = (int)std::min<int64_t>(npairs, std::max<int64_t>(1, g_plain_budget_words / wpp));
     if ((rc = w.p_trace.ensure((size_t)chunk * wpp * 4))) return rc;
-    if ((rc = w.p_res.ensure((size_t)chunk * sizeof(PairResult)))) return rc;
-    if ((rc = w.p_ptr.ensure((size_t)chunk * sizeof(PairTrace)))) return rc;
     for (int c0 = 0; c0 < npairs; c0 += chunk) {
         int cn = std::min(chunk, npairs - c0);
+        PlainOut out{score_d, start_d, end_d, alen_d, qcons_d, rcons_d, (int64_t)qlen + rlen + 2};
         SWIFT_HIP_TRY(launch_plain(q_d, r_d, qlen, rlen, sc, cn, c0, w.p_trace.as<uint32_t>(),
-                                   wpp, w.p_res.as<PairResult>(), w.p_ptr.as<PairTrace>(), st));
-        WalkArgs wa{};
-        wa.res = w.p_res.as<PairResult>();
-        wa.ptr = w.p_ptr.as<PairTrace>();
-        wa.trace = w.p_trace.as<uint32_t>();
-        wa.qseq = q_d;
-        wa.qstride = qlen + 1;
-        wa.rseq = r_d;
-        wa.rstride = rlen + 1;
-        wa.pair0 = c0;
-        wa.npairs = cn;
-        wa.res_global = 0;
-        wa.score_f = score_d;
-        wa.start = start_d;
-        wa.end = end_d;
-        wa.alen = alen_d;
-        wa.qcons = qcons_d;
-        wa.rcons = rcons_d;
-        wa.cstride = qlen + rlen + 2;
-        SWIFT_HIP_TRY(launch_walk(wa, st));
+                                   wpp, out, st));
     }
     return kOk;
 }
@@ -187,7 +200,7 @@ static int max_tasks_for(int npairs) { return (npairs + kSlots32 - 1) / kSlots32
 
 static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
     int R = std::max(1, (maxq + kBatchG - 1) / kBatchG);
-    int64_t steps = maxr + kBatchG - 1;
+    int64_t steps = (maxr + kBatchG - 1 + 3) & ~3;
     int64_t w16 = (R + 7) / 8, w32 = (r32_class(R) + 15) / 16;
     int64_t t16 = (npairs + kSlots16 - 1) / kSlots16 + 2 * kRBins;
     int64_t t32 = (npairs + kSlots32 - 1) / kSlots32 + 2 * kRBins;
@@ -201,6 +214,8 @@ static int reserve_common(Workspace &w, int npairs) {
     if ((rc = w.ptr.ensure((size_t)std::max(npairs, 1) * sizeof(PairTrace)))) return rc;
     if ((rc = w.bin_of.ensure((size_t)std::max(npairs, 1) * 4))) return rc;
     if ((rc = w.order.ensure((size_t)std::max(npairs, 1) * 4))) return rc;
+    if ((rc = w.order2.ensure((size_t)std::max(npairs, 1) * 4))) return rc;
+    if ((rc = ensure_streams(w))) return rc;
     if ((rc = w.tasks.ensure((size_t)mt * sizeof(Task)))) return rc;
     if ((rc = w.task_words.ensure((size_t)mt * 8))) return rc;
     if ((rc = w.bins.ensure((size_t)4 * kNumBins * 4))) return rc;
@@ -264,6 +279,7 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
     // results start at "score 0": a pair no kernel reaches never walks
     SWIFT_HIP_TRY(launch_fill(w.res.as<int32_t>(), n * (int)(sizeof(PairResult) / 4), 0, st));
     SWIFT_HIP_TRY(launch_fill(cnt + kCntError, 1, 0, st));
+    SWIFT_HIP_TRY(mark(0, st));
     // ---- pass 1: bin every pair
     BinArgs b1;
     TaskArgs t1;
@@ -279,32 +295,47 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
     SWIFT_HIP_TRY(launch_fill(cnt + kCntQueues, 2 * kQueuesPerPass, 0, st));
     BatchArgs ba{io.qseq, io.qoff, io.qlen, io.rseq, io.roff, io.rlen, w.order.as<int32_t>(),
                  w.tasks.as<Task>(), nullptr, nullptr, nullptr, w.trace.as<uint32_t>(),
-                 w.res.as<PairResult>(), w.ptr.as<PairTrace>(), kc16, h_limit};
-    const int grid = batch_grid();
-    // one persistent kernel per (class, R) segment, longest rows first; each
-    // reads its task range from the device, so empty segments exit at once
-    auto launch_class = [&](const TaskArgs &t, int cls, int pass) -> int {
-        for (int R = kRBins; R >= 1; --R) {
-            if (cls == 1 && r32_class(R) != R) continue;
-            int seg = cls * kRBins + (kRBins - R);
-            ba.task_lo = t.seg_task0 + seg;
-            ba.task_hi = t.seg_task0 + seg + 1;
-            ba.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
-            ba.kc = cls == 0 ? kc16 : kc32;
-            SWIFT_HIP_TRY(launch_batch(ba, cls == 0, R, grid, st));
+                 w.res.as<PairResult>(), w.ptr.as<PairTrace>(), kc16, h_limit,
+                 io.score, io.start, io.end, io.alen, io.qcons, io.rcons, io.cons_off};
+    // One persistent kernel per (class, R) segment, each reading its task
+    // range from the device (empty segments exit at once), spread longest
+    // rows first over the side streams so the classes run concurrently and
+    // one class's tail overlaps the next class's work.
+    auto launch_classes = [&](const TaskArgs &t, bool with16, int pass) -> int {
+        SWIFT_HIP_TRY(hipEventRecord(w.fork, st));
+        for (int i = 0; i < Workspace::kSide; ++i)
+            SWIFT_HIP_TRY(hipStreamWaitEvent(w.side[i], w.fork, 0));
+        int k = 0;
+        for (int cls = with16 ? 0 : 1; cls < 2; ++cls) {
+            for (int R = kRBins; R >= 1; --R) {
+                if (cls == 1 && r32_class(R) != R) continue;
+                int seg = cls * kRBins + (kRBins - R);
+                BatchArgs b = ba;
+                b.task_lo = t.seg_task0 + seg;
+                b.task_hi = t.seg_task0 + seg + 1;
+                b.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
+                b.kc = cls == 0 ? kc16 : kc32;
+                hipStream_t s2 = w.side[k++ % Workspace::kSide];
+                SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, tuning().waves_per_cu, s2));
+            }
+        }
+        for (int i = 0; i < Workspace::kSide; ++i) {
+            SWIFT_HIP_TRY(hipEventRecord(w.join[i], w.side[i]));
+            SWIFT_HIP_TRY(hipStreamWaitEvent(st, w.join[i], 0));
         }
         return kOk;
     };
-    if (!force32 && (rc = launch_class(t1, 0, 0))) return rc;
-    if ((rc = launch_class(t1, 1, 0))) return rc;
+    SWIFT_HIP_TRY(mark(1, st));
+    if ((rc = launch_classes(t1, !force32, 0))) return rc;
 
+    SWIFT_HIP_TRY(mark(2, st));
     // ---- pass 2: int32 re-run of pairs flagged by the int16 kernel
     int64_t words2 = words1;
     if (!force32) {
         BinArgs b2;
         TaskArgs t2;
         pass_args(1, w.res.as<PairResult>(), 1, b2, t2);
-        SWIFT_HIP_TRY(launch_binning(b2, t2, w.order.as<int32_t>(), w.tasks.as<Task>(),
+        SWIFT_HIP_TRY(launch_binning(b2, t2, w.order2.as<int32_t>(), w.tasks.as<Task>(),
                                      w.task_words.as<int64_t>(), tot + 0, tot + 1, mt, st));
         int32_t nre = 0;
         if (sync_sizes) {
@@ -327,34 +358,15 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
             }
         }
         if (!sync_sizes || nre > 0) {
-            ba.tasks = w.tasks.as<Task>();
-            if ((rc = launch_class(t2, 1, 1))) return rc;
+            ba.order = w.order2.as<int32_t>();
+            if ((rc = launch_classes(t2, false, 1))) return rc;
         }
     }
     w.stats.trace_words = words2;
 
-    // ---- walk-back for every pair
-    WalkArgs wa{};
-    wa.res = w.res.as<PairResult>();
-    wa.ptr = w.ptr.as<PairTrace>();
-    wa.trace = w.trace.as<uint32_t>();
-    wa.qseq = io.qseq;
-    wa.qoff = io.qoff;
-    wa.rseq = io.rseq;
-    wa.roff = io.roff;
-    wa.qlen = io.qlen;
-    wa.rlen = io.rlen;
-    wa.pair0 = 0;
-    wa.npairs = n;
-    wa.res_global = 1;
-    wa.score_i = io.score;
-    wa.start = io.start;
-    wa.end = io.end;
-    wa.alen = io.alen;
-    wa.qcons = io.qcons;
-    wa.rcons = io.rcons;
-    wa.coff = io.cons_off;
-    SWIFT_HIP_TRY(launch_walk(wa, st));
+    // ---- the walk-back ran inside the forward kernels (fused)
+    SWIFT_HIP_TRY(mark(3, st));
+    SWIFT_HIP_TRY(mark(4, st));
     if (sync_sizes) {
         int32_t c[16];
         SWIFT_HIP_TRY(hipMemcpyAsync(c, cnt, 16 * 4, hipMemcpyDeviceToHost, st));
@@ -363,6 +375,10 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
         SWIFT_HIP_TRY(hipMemcpy(segbuf, t1.seg_task0, sizeof(segbuf), hipMemcpyDeviceToHost));
         w.stats.ntasks16 = force32 ? 0 : segbuf[kRBins];
         w.stats.ntasks32 = segbuf[2 * kRBins] - segbuf[kRBins];
+        if (g_timing.on) {
+            for (int i = 0; i < 4; ++i)
+                SWIFT_HIP_TRY(hipEventElapsedTime(&g_timing.ms[i], g_timing.ev[i], g_timing.ev[i + 1]));
+        }
         if (c[kCntError]) {
             set_error("pair lengths beyond maxq/maxr or the batched limits (%d/%d)", kMaxBatchQ,
                       kMaxBatchR);

@@ -8,17 +8,16 @@ namespace swift {
 struct KeyConsts;
 struct Task;
 struct BatchArgs;
-struct WalkArgs;
 struct BinArgs;
 struct TaskArgs;
 
 // launch helpers implemented in sw_kernels.hip
+struct PlainOut;
 hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
-                        int npairs, int pair0, uint32_t *trace, int64_t wpp, PairResult *res,
-                        PairTrace *ptr, hipStream_t st);
+                        int npairs, int pair0, uint32_t *trace, int64_t wpp, const PlainOut &out,
+                        hipStream_t st);
 int plain_rows_for(int qlen);
 int plain_trace_words(int R);
-hipError_t launch_walk(const WalkArgs &wa, hipStream_t st);
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st);
 
 // Device buffer that only grows.
@@ -52,7 +51,6 @@ struct BatchIO {
 int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32);
 int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync_sizes);
 int reserve_batch(int npairs, int maxq, int maxr);
-int batch_grid();
 
 // statistics of the last batched run (for tests and the bench)
 struct BatchStats {
@@ -61,9 +59,14 @@ struct BatchStats {
 };
 BatchStats last_batch_stats();
 
+// HIP-event phase timing of swift_sw_batch (binning, int16/int32 forward
+// sweep, re-run pass, walk-back), in ms, recorded on the call's stream
+void set_batch_timing(bool on);
+const float *batch_timing_ms();
+
 // tuning knobs set by the autotuner (see tune.cpp)
 struct Tuning {
-    int waves_per_cu = 8;       // persistent grid size
+    int waves_per_cu = 0;       // cap on persistent waves per CU (0: occupancy limit)
     int int16 = 1;              // allow the int16 packed path
 };
 Tuning &tuning();

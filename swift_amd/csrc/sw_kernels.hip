@@ -78,6 +78,15 @@ template <bool PK> __device__ __forceinline__ uint32_t vbit(uint32_t k, uint32_t
     else return (k >> s) & 1u;
 }
 
+// Hides a value's range from the optimiser (no instruction is emitted), so
+// that products by 0/1 flags and masked merges stay packed v_pk_* /
+// v_bitop3 arithmetic instead of being rewritten into per-half compares and
+// selects (there is no packed select).
+__device__ __forceinline__ uint32_t opaque(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_shift(uint32_t old, uint32_t src) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, 0xF, 0xF, false);
@@ -99,7 +108,12 @@ __device__ __forceinline__ uint32_t byte_rcode(unsigned char c) {
 template <bool PK, int POL>
 __device__ __forceinline__ uint32_t sub_flag(uint32_t q, uint32_t r) {
     if constexpr (POL == kOneHot) {
-        return vmin1<PK>(q & r);  // 1 = match
+        // q: per element class mask of the query byte, r: class index of the
+        // reference byte; 1 = match
+        uint32_t f;
+        if constexpr (PK) f = as_u(as_us(q) >> as_us(r)) & 0x00010001u;
+        else f = (q >> r) & 1u;
+        return opaque(f);
     } else {
         static_assert(!PK, "byte policy is int32 only");
         uint32_t x = (q ^ r) & ((q & r) >> 8) & 0xFFu;
@@ -127,6 +141,15 @@ struct Sweep {
     uint32_t eu_last, dr_last, dr_top;
 
     uint32_t c_dr, c_dsub, c_eo2, c_eo1, c_de;
+    // PK: byte tables of the gap exits indexed by direction code, laid out
+    // for v_perm_b32 (entry of code c at byte 2c+1, sign via selector 8+c)
+    uint32_t lut_u_lo, lut_u_hi, lut_l_lo, lut_l_hi;
+
+    static __device__ __forceinline__ void lut(int v0, int v1, int v2, int v3, uint32_t &lo,
+                                               uint32_t &hi) {
+        lo = ((uint32_t)(v0 & 0xFF) << 8) | ((uint32_t)(v1 & 0xFF) << 24);
+        hi = ((uint32_t)(v2 & 0xFF) << 8) | ((uint32_t)(v3 & 0xFF) << 24);
+    }
 
     __device__ __forceinline__ void init(const KeyConsts &k) {
         c_dr = splat<PK>(k.dr);
@@ -134,6 +157,10 @@ struct Sweep {
         c_eo2 = splat<PK>(k.eo2);
         c_eo1 = splat<PK>(k.eo1);
         c_de = splat<PK>(k.de);
+        // up exit: open unless the cell came from above (code 2); left exit:
+        // open unless it came from the left (code 1)
+        lut(k.eo2, k.eo2, k.eo2 + k.de, k.eo2, lut_u_lo, lut_u_hi);
+        lut(k.eo1, k.eo1 + k.de, k.eo1, k.eo1, lut_l_lo, lut_l_hi);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             dr[r] = c_dr;   // column 0: H = 0, start
@@ -173,12 +200,19 @@ struct Sweep {
             acc[r / KPW] |= c << (2 * (r % KPW));
             diag = dr[r];
             dr[r] = vadd<PK>(h4, c_dr);
-            eu = vmad<PK>(vbit<PK>(4u, c), c_de, vadd<PK>(h4, c_eo2));
-            el[r] = vmad<PK>(vbit<PK>(2u, c), c_de, vadd<PK>(h4, c_eo1));
+            if constexpr (PK) {
+                // sign-extended per-half gap exit from the byte tables
+                uint32_t sel = c * 0x0102u + 0x08010801u;
+                eu = vadd<true>(h4, __builtin_amdgcn_perm(lut_u_hi, lut_u_lo, sel));
+                el[r] = vadd<true>(h4, __builtin_amdgcn_perm(lut_l_hi, lut_l_lo, sel));
+            } else {
+                eu = vmad<PK>(opaque(vbit<PK>(4u, c)), c_de, vadd<PK>(h4, c_eo2));
+                el[r] = vmad<PK>(opaque(vbit<PK>(2u, c)), c_de, vadd<PK>(h4, c_eo1));
+            }
             if constexpr (PK) {
                 uint32_t hv = MASKED ? (h4 & vmask) : h4;
-                uint32_t d = vsub<true>(best[r], hv);          // < 0 <=> new strict max
-                uint32_t m = as_u(as_s(d) >> (v2s){15, 15});  // 0xFFFF per new max
+                uint32_t d = vsub<true>(best[r], hv);                  // < 0 <=> new strict max
+                uint32_t m = opaque(as_u(as_s(d) >> (v2s){15, 15}));  // 0xFFFF per new max
                 bcol[r] = (m & jj) | (~m & bcol[r]);
                 best[r] = vmax<true>(best[r], hv);
             } else {
@@ -194,13 +228,17 @@ struct Sweep {
 
 // ---------------------------------------------------------------------------
 // Result of a lane group: per lane, best (H, row, col) over its valid rows,
-// then a group-wide max of (H desc, row asc).  Returns through `out`.
+// then a group-wide max of (H desc, row asc) -- the reference's first strict
+// maximum in row-major order.  Every lane of the group receives the result.
 // ---------------------------------------------------------------------------
+struct GroupResult {
+    int32_t score, row, col, flag;
+};
+
 template <int G>
-__device__ __forceinline__ void group_best(uint32_t lane_h, uint32_t lane_row, uint32_t lane_col,
-                                           int g, PairResult *out, bool flag_overflow,
-                                           uint32_t h_limit) {
-    // key: H high, then smaller row first
+__device__ __forceinline__ GroupResult group_best(uint32_t lane_h, uint32_t lane_row,
+                                                  uint32_t lane_col, int R, bool flag_overflow,
+                                                  uint32_t h_limit, uint32_t force_flag) {
     uint64_t key = lane_h ? (((uint64_t)lane_h << 32) | (0xFFFFFFFFu - lane_row)) : 0;
     uint64_t m = key;
 #pragma unroll
@@ -208,12 +246,130 @@ __device__ __forceinline__ void group_best(uint32_t lane_h, uint32_t lane_row, u
         uint64_t o = __shfl_xor(m, s, G);
         m = o > m ? o : m;
     }
-    if (m == 0) {
-        if (g == 0 && out) *out = PairResult{0, 0, 0, 0};
-    } else if (key == m && out) {
-        int fl = (flag_overflow && lane_h * 4 >= h_limit) ? 1 : 0;
-        *out = PairResult{(int32_t)lane_h, (int32_t)lane_row, (int32_t)lane_col, fl};
+    GroupResult gr;
+    gr.score = (int32_t)(m >> 32);
+    gr.row = m ? (int32_t)(0xFFFFFFFFu - (uint32_t)m) : 0;
+    // the column lives in the lane that owns the best row
+    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+    const int src = base + (gr.row > 0 ? (gr.row - 1) / R : 0);
+    gr.col = m ? __shfl((int)lane_col, src, 64) : 0;
+    gr.flag = ((flag_overflow && (uint32_t)gr.score * 4 >= h_limit) || force_flag) ? 1 : 0;
+    return gr;
+}
+
+// Output of one consensus string, four bytes per store when its start is
+// 4-byte aligned.
+struct ConsWriter {
+    char *base;
+    uint32_t word = 0;
+    int n = 0;
+    bool aligned;
+    __device__ explicit ConsWriter(char *b) : base(b), aligned(((uintptr_t)b & 3) == 0) {}
+    __device__ __forceinline__ void put(char c) {
+        if (aligned) {
+            word |= (uint32_t)(unsigned char)c << (8 * (n & 3));
+            if ((n & 3) == 3) {
+                *(uint32_t *)(base + (n & ~3)) = word;
+                word = 0;
+            }
+        } else {
+            base[n] = c;
+        }
+        ++n;
     }
+    __device__ __forceinline__ void finish() {  // NUL terminator
+        put('\0');
+        if (aligned && (n & 3)) *(uint32_t *)(base + (n & ~3)) = word;
+    }
+};
+
+// Trace layout: granules of 4 steps, [t/4][w][lane][t%4] (16 bytes per lane
+// and word index), so a walk that stays on one lane for a few columns reads
+// one granule instead of one cache line per step.
+struct TraceReader {
+    const uint4 *tr;
+    PairTrace pt;
+    int64_t cached = -1;
+    uint4 gran;
+    __device__ __forceinline__ int code(int i, int j) {
+        const int R = pt.rows;
+        const int kpw = (pt.half & 2) ? 8 : 16;
+        const int g = (i - 1) / R, k = (i - 1) - g * R;
+        const int lane = pt.lane0 + g, t = (j - 1) + g;
+        const int w = k / kpw;
+        const int bit = (pt.half & 1) * 16 + 2 * (k - w * kpw);
+        const int64_t gi = pt.base / 4 + ((int64_t)(t >> 2) * pt.words + w) * 64 + lane;
+        if (gi != cached) {
+            gran = tr[gi];
+            cached = gi;
+        }
+        const int u = t & 3;
+        uint32_t word = u == 0 ? gran.x : u == 1 ? gran.y : u == 2 ? gran.z : gran.w;
+        return (word >> bit) & 3;
+    }
+};
+
+// Word offset of step t, trace word w, lane `lane` in the granule layout.
+template <int W>
+__device__ __forceinline__ int64_t trace_word(int t, int w, int lane) {
+    return (((int64_t)(t >> 2) * W + w) * 64 + lane) * 4 + (t & 3);
+}
+
+// Runs steps [0, T): masked before lo and from hi on, unmasked between.
+template <class Run>
+__device__ __forceinline__ void sweep_steps(int T, int lo, int hi, Run &run) {
+    int t = 0;
+    for (; t < lo; ++t) run(std::true_type{}, t);
+    for (; t < hi; ++t) run(std::false_type{}, t);
+    for (; t < T; ++t) run(std::true_type{}, t);
+}
+
+struct WalkOut {
+    int start, end, len;
+};
+
+// backtrack_cpu (src/smithWaterman.cu:263-335) over the packed direction
+// codes: emit the current cell while its pointer leads elsewhere; a pointer
+// into row 0 or column 0 emits the current cell and stops; a start pointer
+// stops without emitting.  Strings are written in walk order (the reference
+// stores them reversed) and NUL terminated.
+__device__ WalkOut walk_pair(const uint32_t *trace, int score, int row, int col,
+                             const PairTrace &pt, const char *qs, const char *rs, int qlen,
+                             int rlen, char *qcons, char *rcons) {
+    ConsWriter qc(qcons), rc(rcons);
+    int start = 0, nrb = -1, tick = 0;
+    if (score > 0 && row >= 1 && row <= qlen && col >= 1 && col <= rlen) {
+        int ci = row, cj = col, ni, nj;
+        TraceReader rd{reinterpret_cast<const uint4 *>(trace), pt};
+        auto back = [&](int i, int j, int &oi, int &oj) {
+            int c = rd.code(i, j);
+            oi = (c & 2) ? i - 1 : i;            // diag (3) and up (2) move up
+            oj = (c == 3 || c == 1) ? j - 1 : j;  // diag (3) and left (1) move left
+        };
+        back(ci, cj, ni, nj);
+        for (;;) {
+            if (ci == ni && cj == nj) break;
+            if (nj == cj) rc.put('-');
+            else { rc.put(rs[cj - 1]); ++nrb; }
+            qc.put((ni == ci) ? '-' : qs[ci - 1]);
+            start = cj;
+            ++tick;
+            if (ni == 0 || nj == 0) break;  // the target is in row/column 0
+            ci = ni;
+            cj = nj;
+            back(ci, cj, ni, nj);
+        }
+    }
+    qc.finish();
+    rc.finish();
+    return WalkOut{start, start + nrb, tick};
+}
+
+// Makes this wave's earlier global stores (trace words) visible to its own
+// later loads from other lanes.
+__device__ __forceinline__ void wave_store_fence() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 }
 
 // ---------------------------------------------------------------------------
@@ -227,8 +383,7 @@ template <int R>
 __global__ void __launch_bounds__(64)
 sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, int qlen, int rlen_max,
                 KeyConsts kc, int npairs, int pair0, uint32_t *__restrict__ trace,
-                int64_t trace_words_per_pair, PairResult *__restrict__ res,
-                PairTrace *__restrict__ ptr) {
+                int64_t trace_words_per_pair, PlainOut out) {
     using S = Sweep<64, R, false, kBytes>;
     extern __shared__ uint32_t lds_rr[];
     const int p = blockIdx.x;
@@ -259,21 +414,18 @@ sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, in
     uint32_t *tr = trace + (int64_t)p * trace_words_per_pair;
     const int T = rlen + 63;
     const int main_lo = 63, main_hi = rlen;  // every lane inside [0, rlen)
-    uint32_t acc[S::W];
     auto run = [&](auto masked, int t) {
         int j = t - g;
         uint32_t rr = (j >= 0 && j < rlen) ? lds_rr[j] : 0u;
         uint32_t vmask = (j >= 0 && j < rlen) ? ~0u : 0u;
+        uint32_t a1[S::W];
 #pragma unroll
-        for (int w = 0; w < S::W; ++w) acc[w] = 0;
-        sw.template step<decltype(masked)::value>(rr, j, vmask, acc);
+        for (int w = 0; w < S::W; ++w) a1[w] = 0;
+        sw.template step<decltype(masked)::value>(rr, j, vmask, a1);
 #pragma unroll
-        for (int w = 0; w < S::W; ++w) tr[((int64_t)t * S::W + w) * 64 + g] = acc[w];
+        for (int w = 0; w < S::W; ++w) tr[trace_word<S::W>(t, w, g)] = a1[w];
     };
-    int t = 0;
-    for (; t < min(main_lo, T); ++t) run(std::true_type{}, t);
-    for (; t < main_hi; ++t) run(std::false_type{}, t);
-    for (; t < T; ++t) run(std::true_type{}, t);
+    sweep_steps(T, min(main_lo, T), main_hi, run);
 
     // lane best over its valid rows: row-major first maximum
     uint32_t bh = 0, brow = 0, bcol = 0;
@@ -287,14 +439,27 @@ sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, in
             bcol = kMaxCols - (sw.best[r] & kMaxCols) + 1;
         }
     }
-    group_best<64>(bh, brow, bcol, g, res + p, false, 0);
-    if (g == 0)
-        ptr[p] = PairTrace{(int64_t)p * trace_words_per_pair, 0, (int16_t)R, (uint8_t)S::W, 0};
+    const GroupResult gr = group_best<64>(bh, brow, bcol, R, false, 0, 0);
+    wave_store_fence();
+    if (g == 0) {  // one lane walks the traceback, as the reference's thread 0 does
+        const int gp = pair0 + p;
+        const PairTrace pt{(int64_t)p * trace_words_per_pair, 0, (int16_t)R, (uint8_t)S::W, 0};
+        char *qc = out.qcons + (int64_t)gp * out.cstride;
+        char *rc = out.rcons + (int64_t)gp * out.cstride;
+        WalkOut wo = walk_pair(trace, gr.score, gr.row, gr.col, pt, qs, rs, qlen, rlen, qc, rc);
+        out.score[gp] = (float)gr.score;
+        out.start[gp] = wo.start;
+        out.end[gp] = wo.end;
+        out.alen[gp] = wo.len;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Batched persistent kernels.
 // ---------------------------------------------------------------------------
+constexpr int kPad = kBatchG;                        // zero columns before column 0
+constexpr int kGroupLds = kPad + kBatchCols + kBatchG; // LDS words per lane group
+
 // One wave task: 4 groups of 16 lanes; PK -> 2 pairs per group (halves).
 template <int R, bool PK, int POL>
 __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uint32_t *lds) {
@@ -302,7 +467,9 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     constexpr int PPG = PK ? 2 : 1;  // pairs per group
     const int lane = threadIdx.x & 63;
     const int grp = lane >> 4, g = lane & 15;
-    uint32_t *rr_lds = lds + grp * kBatchCols;
+    // reference codes at [kPad + j]; the kPad leading entries and the tail
+    // are read by lanes outside the pair's columns (their results unused)
+    uint32_t *rr_lds = lds + grp * kGroupLds;
 
     // the group's pairs
     int pid[PPG], ql[PPG], rl[PPG];
@@ -318,18 +485,26 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     }
     // reference codes of the group's pairs into LDS (both halves packed)
     const int ncols = tk.steps - (kBatchG - 1);
+    uint32_t exotic = 0;  // bit h: the pair's reference holds a byte outside ACGTNacgtn
     for (int j = g; j < ncols; j += kBatchG) {
         uint32_t v = 0;
 #pragma unroll
         for (int h = 0; h < PPG; ++h) {
+            uint32_t code = POL == kOneHot ? kRefOther : 0u;
             if (j < rl[h]) {
                 unsigned char c = (unsigned char)a.rseq[ro[h] + j];
-                uint32_t code = POL == kOneHot ? ref_code(c) : byte_rcode(c);
-                v |= code << (16 * h);
+                code = POL == kOneHot ? ref_index(c) : byte_rcode(c);
+                if (POL == kOneHot && code == kRefOther) exotic |= 1u << h;
             }
+            v |= code << (16 * h);
         }
-        rr_lds[j] = v;
+        rr_lds[kPad + j] = v;
     }
+    if constexpr (POL == kOneHot) {
+#pragma unroll
+        for (int s2 = kBatchG / 2; s2 >= 1; s2 >>= 1) exotic |= __shfl_xor(exotic, s2, kBatchG);
+    }
+    if (g < kPad) rr_lds[g] = POL == kOneHot ? (PK ? 0x00050005u : kRefOther) : 0u;
     S sw;
     sw.init(a.kc);
 #pragma unroll
@@ -340,7 +515,7 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
         for (int h = 0; h < PPG; ++h) {
             if (row < ql[h]) {
                 unsigned char c = (unsigned char)a.qseq[qo[h] + row];
-                uint32_t code = POL == kOneHot ? query_code(c) : byte_qcode(c);
+                uint32_t code = POL == kOneHot ? query_mask(c) : byte_qcode(c);
                 v |= code << (16 * h);
             }
         }
@@ -351,32 +526,32 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     __builtin_amdgcn_wave_barrier();
 
     uint32_t *tr = a.trace + tk.trace_base;
-    uint32_t vm_full = 0, acc[S::W];
+    uint32_t rr_next = rr_lds[kPad - g];  // column of step 0
     auto run = [&](auto masked, int t) {
         int j = t - g;
-        bool in = j >= 0 && j < ncols;
-        uint32_t rr = in ? rr_lds[j] : 0u;
+        uint32_t rr = rr_next;
+        // prefetch the next step's column (clamped: steps past the last
+        // granule boundary read a stale word whose results are unused)
+        rr_next = rr_lds[min(kPad + 1 + j, kGroupLds - 1)];
         uint32_t vmask = 0;
         if constexpr (decltype(masked)::value) {
 #pragma unroll
             for (int h = 0; h < PPG; ++h)
                 if (j >= 0 && j < rl[h]) vmask |= (PK ? 0xFFFFu : 0xFFFFFFFFu) << (16 * h);
         }
+        uint32_t a1[S::W];
 #pragma unroll
-        for (int w = 0; w < S::W; ++w) acc[w] = 0;
-        sw.template step<decltype(masked)::value>(rr, j, vmask, acc);
+        for (int w = 0; w < S::W; ++w) a1[w] = 0;
+        sw.template step<decltype(masked)::value>(rr, j, vmask, a1);
 #pragma unroll
-        for (int w = 0; w < S::W; ++w) tr[((int64_t)t * S::W + w) * 64 + lane] = acc[w];
+        for (int w = 0; w < S::W; ++w) tr[trace_word<S::W>(t, w, lane)] = a1[w];
     };
-    (void)vm_full;
     const int T = tk.steps;
     const int lo = min(kBatchG - 1, T), hi = max(lo, tk.min_rlen);
-    int t = 0;
-    for (; t < lo; ++t) run(std::true_type{}, t);
-    for (; t < hi; ++t) run(std::false_type{}, t);
-    for (; t < T; ++t) run(std::true_type{}, t);
+    sweep_steps(T, lo, hi, run);
 
     // per half: lane best over valid rows, then group max
+    GroupResult gres[PPG];
 #pragma unroll
     for (int h = 0; h < PPG; ++h) {
         uint32_t bh = 0, brow = 0, bcol = 0;
@@ -397,11 +572,43 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
                 bcol = cv;
             }
         }
-        PairResult *o = pid[h] >= 0 ? a.res + pid[h] : nullptr;
-        group_best<kBatchG>(bh, brow, bcol, g, o, PK, a.h_limit);
+        gres[h] = group_best<kBatchG>(bh, brow, bcol, R, PK, a.h_limit, (exotic >> h) & 1);
         if (g == 0 && pid[h] >= 0)
-            a.ptr[pid[h]] = PairTrace{tk.trace_base, grp * kBatchG, (int16_t)R, (uint8_t)S::W,
-                                      (int8_t)(h | (PK ? 2 : 0))};
+            a.res[pid[h]] = PairResult{gres[h].score, gres[h].row, gres[h].col, gres[h].flag};
+    }
+
+    // Traceback of the task's pairs right after the sweep, while the trace
+    // is cache-hot: lane s walks slot s.  A flagged pair is left to the int32
+    // re-run.  The walk is latency-bound; other waves' sweeps cover it.
+    wave_store_fence();
+    const int s = lane;
+    const int sg = min(s / PPG, 3), sh = s % PPG;
+    int w_score = 0, w_row = 0, w_col = 0, w_flag = 1;
+#pragma unroll
+    for (int h = 0; h < PPG; ++h) {
+        int v0 = __shfl(gres[h].score, sg * kBatchG, 64);
+        int v1 = __shfl(gres[h].row, sg * kBatchG, 64);
+        int v2 = __shfl(gres[h].col, sg * kBatchG, 64);
+        int v3 = __shfl(gres[h].flag, sg * kBatchG, 64);
+        if (h == sh) {
+            w_score = v0;
+            w_row = v1;
+            w_col = v2;
+            w_flag = v3;
+        }
+    }
+    if (s < tk.npairs && !w_flag) {
+        const int p = a.order[tk.first + s];
+        const PairTrace pt{tk.trace_base, sg * kBatchG, (int16_t)R, (uint8_t)S::W,
+                           (int8_t)(sh | (PK ? 2 : 0))};
+        const int64_t co = a.coff[p];
+        WalkOut wo = walk_pair(a.trace, w_score, w_row, w_col, pt, a.qseq + a.qoff[p],
+                               a.rseq + a.roff[p], a.qlen[p], a.rlen[p], a.qcons + co,
+                               a.rcons + co);
+        a.score_out[p] = w_score;
+        a.start_out[p] = wo.start;
+        a.end_out[p] = wo.end;
+        a.alen_out[p] = wo.len;
     }
     __builtin_amdgcn_wave_barrier();
 }
@@ -413,10 +620,17 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
 #define SWIFT_R32_CASES(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20)
 
+// Occupancy floor per row class (waves per SIMD), set where the natural
+// register count sits just above an occupancy step.
+__host__ __device__ constexpr int batch_waves_per_eu(int R, bool pk) {
+    return pk ? (R <= 7 ? 5 : R <= 12 ? 4 : R <= 16 ? 3 : 2) : (R <= 10 ? 5 : R <= 16 ? 4 : 3);
+}
+
 template <int R, bool PK, int POL>
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(batch_waves_per_eu(R, PK))))
 sw_batch_kernel(BatchArgs a) {
-    __shared__ uint32_t lds[4][4 * kBatchCols];
+    __shared__ uint32_t lds[4][4 * kGroupLds];
     const int wave = threadIdx.x >> 6;
     const int lo = *a.task_lo, hi = *a.task_hi;
     if (lo >= hi) return;
@@ -428,69 +642,6 @@ sw_batch_kernel(BatchArgs a) {
         const Task tk = a.tasks[tid];
         run_task<R, PK, POL>(a, tk, lds[wave]);
     }
-}
-
-// ---------------------------------------------------------------------------
-// Walk-back: one lane per pair (backtrack_cpu, src/smithWaterman.cu:263-335).
-// Emits the consensus strings in walk order (the reference stores them
-// reversed), start/end (1-based window columns) and the alignment length.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int trace_code(const uint32_t *tr, const PairTrace &pt, int i, int j) {
-    const int R = pt.rows;
-    const int kpw = (pt.half & 2) ? 8 : 16;
-    const int g = (i - 1) / R, k = (i - 1) - g * R;
-    const int lane = pt.lane0 + g, t = (j - 1) + g;
-    const int w = k / kpw;
-    const int bit = (pt.half & 1) * 16 + 2 * (k - w * kpw);
-    uint32_t word = tr[pt.base + ((int64_t)t * pt.words + w) * 64 + lane];
-    return (word >> bit) & 3;
-}
-
-__global__ void __launch_bounds__(256)
-sw_walk_kernel(WalkArgs a) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npairs) return;
-    const int gp = a.pair0 + p;
-    const int ri = a.res_global ? gp : p;
-    const PairResult r = a.res[ri];
-    const char *qs = a.qseq + (a.qoff ? a.qoff[gp] : (int64_t)gp * a.qstride);
-    const char *rs = a.rseq + (a.roff ? a.roff[gp] : (int64_t)gp * a.rstride);
-    const int64_t co = a.coff ? a.coff[gp] : (int64_t)gp * a.cstride;
-    char *qc = a.qcons + co;
-    char *rc = a.rcons + co;
-    int start = 0, nrb = -1, tick = 0;
-    const int qlen = a.qlen ? a.qlen[gp] : (int)a.qstride - 1;
-    const int rlen = a.rlen ? a.rlen[gp] : (int)a.rstride - 1;
-    if (r.score > 0 && r.row >= 1 && r.row <= qlen && r.col >= 1 && r.col <= rlen) {
-        const PairTrace pt = a.ptr[ri];
-        int ci = r.row, cj = r.col, ni, nj;
-        auto back = [&](int i, int j, int &oi, int &oj) {
-            int c = trace_code(a.trace, pt, i, j);
-            oi = (c & 2) ? i - 1 : i;           // diag (3) and up (2) move up
-            oj = (c == 3 || c == 1) ? j - 1 : j; // diag (3) and left (1) move left
-        };
-        back(ci, cj, ni, nj);
-        for (;;) {
-            bool moves = ci != ni || cj != nj;
-            if (!moves) break;
-            if (nj == cj) rc[tick] = '-';
-            else { rc[tick] = rs[cj - 1]; ++nrb; }
-            qc[tick] = (ni == ci) ? '-' : qs[ci - 1];
-            start = cj;
-            ++tick;
-            if (ni == 0 || nj == 0) break;  // the target is in row/column 0
-            ci = ni;
-            cj = nj;
-            back(ci, cj, ni, nj);
-        }
-    }
-    qc[tick] = '\0';
-    rc[tick] = '\0';
-    if (a.score_f) a.score_f[gp] = (float)r.score;
-    if (a.score_i) a.score_i[gp] = r.score;
-    a.start[gp] = start;
-    a.end[gp] = start + nrb;
-    a.alen[gp] = tick;
 }
 
 // ---------------------------------------------------------------------------
@@ -518,12 +669,9 @@ __global__ void sw_bin_classify(BinArgs a) {
         if (!(a.prev[p].flags & 1)) { a.bin_of[p] = -1; return; }
         cls = 1;
     } else {
+        // references with bytes outside ACGTNacgtn are caught by the int16
+        // kernel itself and re-run with byte semantics in the int32 pass
         cls = a.force32 ? 1 : 0;
-        if (!cls) {
-            const char *r = a.rseq + a.roff[p];
-            for (int j = 0; j < rl; ++j)
-                if (ref_code((unsigned char)r[j]) == 0) { cls = 1; break; }
-        }
     }
     int R = max(1, (ql + kBatchG - 1) / kBatchG);
     if (cls == 1) R = r32_class(R);
@@ -623,7 +771,7 @@ __global__ void sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_firs
     tasks[t] = tk;
     int kpw = cls == 0 ? 8 : 16;
     int W = (R + kpw - 1) / kpw;
-    task_words[t] = (int64_t)tk.steps * W * 64;
+    task_words[t] = (int64_t)((tk.steps + 3) & ~3) * W * 64;  // whole 4-step granules
 }
 
 // single workgroup exclusive scan of task trace sizes into the descriptors
@@ -669,10 +817,10 @@ __global__ void fill_i32(int32_t *p, int n, int32_t v) {
 template <int R>
 static hipError_t launch_plain_r(const char *q, const char *r, int qlen, int rlen_max,
                                  const KeyConsts &kc, int npairs, int pair0, uint32_t *trace,
-                                 int64_t wpp, PairResult *res, PairTrace *ptr, hipStream_t st) {
+                                 int64_t wpp, const PlainOut &out, hipStream_t st) {
     size_t lds = sizeof(uint32_t) * (size_t)max(rlen_max, 1);
     hipLaunchKernelGGL(sw_plain_kernel<R>, dim3(npairs), dim3(64), lds, st, q, r, qlen, rlen_max,
-                       kc, npairs, pair0, trace, wpp, res, ptr);
+                       kc, npairs, pair0, trace, wpp, out);
     return hipGetLastError();
 }
 
@@ -689,12 +837,12 @@ int plain_rows_for(int qlen) {
 int plain_trace_words(int R) { return (R + 15) / 16; }
 
 hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
-                        int npairs, int pair0, uint32_t *trace, int64_t wpp, PairResult *res,
-                        PairTrace *ptr, hipStream_t st) {
+                        int npairs, int pair0, uint32_t *trace, int64_t wpp, const PlainOut &out,
+                        hipStream_t st) {
     KeyConsts kc = make_consts(sc, kBytes);
     switch (plain_rows_for(qlen)) {
 #define SWIFT_PCASE(RR) \
-    case RR: return launch_plain_r<RR>(q, r, qlen, rlen_max, kc, npairs, pair0, trace, wpp, res, ptr, st);
+    case RR: return launch_plain_r<RR>(q, r, qlen, rlen_max, kc, npairs, pair0, trace, wpp, out, st);
         SWIFT_PCASE(1) SWIFT_PCASE(2) SWIFT_PCASE(3) SWIFT_PCASE(4) SWIFT_PCASE(5)
         SWIFT_PCASE(6) SWIFT_PCASE(7) SWIFT_PCASE(8) SWIFT_PCASE(10) SWIFT_PCASE(12)
         SWIFT_PCASE(16)
@@ -703,25 +851,46 @@ hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, co
     }
 }
 
-hipError_t launch_walk(const WalkArgs &wa, hipStream_t st) {
-    if (wa.npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sw_walk_kernel, dim3((wa.npairs + 255) / 256), dim3(256), 0, st, wa);
-    return hipGetLastError();
+
+// Persistent grid of one row-class kernel: as many 256-thread blocks as fit
+// on every CU at once (cached per kernel); `waves_per_cu` > 0 caps it.
+template <int RR, bool PK, class K>
+static int full_grid(K kernel, int cu_count, int waves_per_cu) {
+    static int cached = -1;  // one per row class and lane width
+    if (cached < 0) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 256, 0) != hipSuccess ||
+            blocks < 1)
+            blocks = 2;
+        cached = blocks;
+    }
+    int b = cached;
+    if (waves_per_cu > 0) b = min(b, max(1, waves_per_cu / 4));
+    return b * cu_count;
 }
 
-hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int grid, hipStream_t st) {
+hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
+                        hipStream_t st) {
     if (int16) {
         switch (rows) {
-#define SWIFT_L16(RR) \
-    case RR: hipLaunchKernelGGL((sw_batch_kernel<RR, true, kOneHot>), dim3(grid), dim3(256), 0, st, ba); break;
+#define SWIFT_L16(RR)                                                                      \
+    case RR: {                                                                             \
+        auto k = sw_batch_kernel<RR, true, kOneHot>;                                       \
+        hipLaunchKernelGGL(k, dim3(full_grid<RR, true>(k, cu_count, waves_per_cu)), dim3(256), 0, st, ba); \
+        break;                                                                             \
+    }
             SWIFT_R16_CASES(SWIFT_L16)
 #undef SWIFT_L16
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (rows) {
-#define SWIFT_L32(RR) \
-    case RR: hipLaunchKernelGGL((sw_batch_kernel<RR, false, kBytes>), dim3(grid), dim3(256), 0, st, ba); break;
+#define SWIFT_L32(RR)                                                                      \
+    case RR: {                                                                             \
+        auto k = sw_batch_kernel<RR, false, kBytes>;                                       \
+        hipLaunchKernelGGL(k, dim3(full_grid<RR, false>(k, cu_count, waves_per_cu)), dim3(256), 0, st, ba); \
+        break;                                                                             \
+    }
             SWIFT_R32_CASES(SWIFT_L32)
 #undef SWIFT_L32
         default: return hipErrorInvalidValue;

@@ -38,12 +38,12 @@ constexpr int kColBits = 12;                   // int32 best key: H << 12 | (409
 constexpr int kMaxCols = (1 << kColBits) - 1;  // longest reference window (plain kernel)
 constexpr int kMaxScore32 = 1 << 19;           // scores must stay below this
 constexpr int kBatchG = 16;                    // lanes per pair group (batched kernels)
-constexpr int kBatchCols = 512;                // reference columns per group (LDS)
+constexpr int kBatchCols = 480;                // reference columns per group (LDS)
 constexpr int kRBins = 20;                     // R = 1..20 rows per lane (query <= 320)
 constexpr int kLenBins = kBatchCols / 8;       // bins of 8 reference columns
 constexpr int kNumBins = 2 * kRBins * kLenBins;
 constexpr int kMaxBatchQ = kBatchG * kRBins;   // 320
-constexpr int kMaxBatchR = kBatchCols - 1;     // 511
+constexpr int kMaxBatchR = kBatchCols;         // 480
 constexpr int kSlots16 = 8;                    // pairs per wave task, int16 packed
 constexpr int kSlots32 = 4;                    // pairs per wave task, int32
 
@@ -75,32 +75,20 @@ struct BatchArgs {
     PairTrace *ptr;
     KeyConsts kc;
     uint32_t h_limit;         // 4*H at or above this flags an int16 overflow risk
+    // traceback outputs, written by the fused walk
+    int32_t *score_out, *start_out, *end_out, *alen_out;
+    char *qcons, *rcons;
+    const int64_t *coff;
 };
 
-struct WalkArgs {
-    const PairResult *res;
-    const PairTrace *ptr;
-    const uint32_t *trace;
-    // sequences: per pair offsets, or fixed strides when the offsets are null
-    const char *qseq;
-    const int64_t *qoff;
-    int64_t qstride;
-    const char *rseq;
-    const int64_t *roff;
-    int64_t rstride;
-    const int32_t *qlen;     // per pair lengths (null: strides - 1)
-    const int32_t *rlen;
-    int pair0;               // first pair of this launch; res/ptr are indexed from it
-    int npairs;
-    int res_global;          // 1: res/ptr indexed by the global pair id
-    // outputs
-    float *score_f;          // drop-in layout (float score) or
-    int32_t *score_i;        // batched layout (int score)
+// outputs of the plain (alignSequences) kernel: reference layout
+struct PlainOut {
+    float *score;
     int32_t *start, *end, *alen;
     char *qcons, *rcons;
-    const int64_t *coff;     // per pair consensus offset, or fixed stride
     int64_t cstride;
 };
+
 
 struct BinArgs {
     const char *rseq;
@@ -126,7 +114,8 @@ struct TaskArgs {
 };
 
 // launch helpers (sw_kernels.hip)
-hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int grid, hipStream_t st);
+hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
+                        hipStream_t st);
 hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order, Task *tasks,
                           int64_t *task_words, const int64_t *base0, int64_t *total_words,
                           int max_tasks, hipStream_t st);

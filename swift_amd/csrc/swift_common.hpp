@@ -79,6 +79,32 @@ __host__ __device__ inline uint32_t query_code(unsigned char c) {
     }
 }
 
+// Shift form of the same rule used by the packed kernels: a reference byte
+// becomes a class index (A 0, C 1, G 2, T 3, N 4, anything else 5) and a
+// query byte a bit mask over the classes it matches, so
+// match(q, r) <=> (query_mask(q) >> ref_index(r)) & 1.
+constexpr uint32_t kRefOther = 5;
+__host__ __device__ inline uint32_t ref_index(unsigned char c) {
+    switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    case 'N': case 'n': return 4;
+    default: return kRefOther;
+    }
+}
+__host__ __device__ inline uint32_t query_mask(unsigned char c) {
+    switch (c) {
+    case 'A': return 0x11;
+    case 'C': return 0x12;
+    case 'G': return 0x14;
+    case 'T': return 0x18;
+    case 'N': return 0x3F;
+    default: return 0x10;  // only a reference N matches
+    }
+}
+
 // 0 marks a reference byte the one-hot kernels cannot represent.
 __host__ __device__ inline uint32_t ref_code(unsigned char c) {
     switch (c) {
