@@ -114,6 +114,123 @@ int swift_sw_batch_timing(int enable, float *ms4);
  * the int16 path may be used. */
 int swift_set_tuning(int waves_per_cu, int allow_int16);
 
+/* ======================================================== phase 1: seeding
+ * Reference drop-ins.  Same names, arguments and meaning as the reference's
+ * CPU phase 1; the index they share lives in device memory. */
+
+/* Query record (src/query.h:29-37). */
+typedef struct queryStruct {
+    char *name;
+    int length;
+    char *seq;
+    char *revCompSeq;
+    unsigned long nameOffset;
+    unsigned long seqOffset;
+} Query;
+
+/* preprocessCreate / preprocessDelete (src/preprocess.h:21,29).  The codes
+ * are fixed tables here; the calls exist for callers that make them. */
+void preprocessCreate(void);
+void preprocessDelete(void);
+/* getHash (src/preprocess.h:23, src/preprocess.cu:70-91): sum_i 4^i code(s[i]),
+ * A/a 0, C/c 1, G/g 2, T/t 3, any other byte 0. */
+int getHash(char *s, int length);
+/* qryGetReverseComplement (src/query.cu:265-282): reversed, A->T C->G G->C,
+ * any other byte -> A; revComp receives seqLength+1 bytes (NUL). */
+void qryGetReverseComplement(char *seq, int seqLength, char *revComp);
+
+/* lookupTable5Create (src/lookupTable5.h:28-29): reads the reference FASTA
+ * (any line width) and builds the non-overlapping k-mer index on the device;
+ * tuples seen more than tupleIgnoreThreshold times are dropped.
+ * *totalTuples receives the number of kept tuples.  Exits on error, like the
+ * reference. */
+void lookupTable5Create(const char *refFile, int seedLen, int maxHitsPerQry,
+                        int tupleIgnoreThreshold, int *totalTuples);
+/* lookupTable5MapQry (src/lookupTable5.h:42-43): best hits of one strand
+ * (query->seq, or query->revCompSeq when isRevComp == 1) over qryLen bases;
+ * returns the number of hits written (<= maxHitsPerQry).  Thread safe: calls
+ * from several host threads are serialised. */
+int lookupTable5MapQry(const Query *query, unsigned int qryLen, unsigned int isRevComp,
+                       char *refIdx_bestHits, int *shift_bestHits, int *refPos_bestHits);
+/* lookupTable5Delete (src/lookupTable5.h:40). */
+void lookupTable5Delete(void);
+/* mapHits6GetBestHits (src/mapHits6.h:34): sorts the size hits in place by
+ * (refIdx, shift, refPos), clusters equal (refIdx, shift) runs and writes the
+ * first hit of each biggest cluster, at most numBestHits of them in sorted
+ * order.  Where the reference draws among more tied clusters with rand(),
+ * this keeps the first numBestHits.  Returns the count. */
+int mapHits6GetBestHits(int numBestHits, char *refIdx, int *shift, int *refPos, int size,
+                        char *refIdx_bestMatch, int *shift_bestMatch, int *refPos_bestMatch);
+/* searchQueries3 (src/search.h:33-35): builds the index from refFileName,
+ * maps both strands of the first numQrs reads of qryFileName on the device
+ * and writes one match line per hit to matchFile,
+ * "name\tstrand\trefIdx\tshift\trefPos\tnameOffset\tseqOffset" (reads in file
+ * order, forward strand first).  qryLen and numThreads are accepted and
+ * ignored.  *numMatches receives the line count. */
+void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName, int seedLen,
+                    char *matchFile, unsigned int maxNumHits, int *numMatches, int numThreads,
+                    int tupleIgnoreThres);
+
+/* ---- status-returning phase-1 API (no reference counterpart) ----------- */
+typedef struct swift_index swift_index;
+
+/* Index over nseq sequences in host memory (sequence s = seq[off[s] ..
+ * +len[s])), k = seed_len <= 15, nseq <= 127. */
+int swift_index_build(const char *seq, const int64_t *off, const int64_t *len, int nseq,
+                      int seed_len, int tuple_ignore_threshold, swift_index **out);
+/* Index over a FASTA file read as lookupTable5Create reads it. */
+int swift_index_build_fasta(const char *path, int seed_len, int tuple_ignore_threshold,
+                            swift_index **out);
+int swift_index_free(swift_index *ix);
+/* nseq, kept tuples, 4^k and the largest kept bucket. */
+int swift_index_info(const swift_index *ix, int *nseq, int64_t *total_tuples, int64_t *nhash,
+                     int *max_bucket);
+/* Copies the tables to host memory (any pointer may be NULL): start, count
+ * and actual ([4^k]: _lookupTable, _numRepeatsPerTuple,
+ * _numActualRepeatsPerTuple) and ref_idx/ref_pos ([total + 1], element 0
+ * the -1 sentinel: _refIdx, _refPos). */
+int swift_index_export(const swift_index *ix, int32_t *start, int32_t *count, int32_t *actual,
+                       int8_t *ref_idx, int32_t *ref_pos);
+/* Length of sequence s as indexed, and its name (FASTA builds; "" otherwise). */
+int swift_index_sequence(const swift_index *ix, int s, int64_t *len, const char **name);
+
+/* Maps both strands of nreads device-resident reads (read r =
+ * reads_d[off_d[r] .. +len_d[r]), len <= max_read_len <= 2047).  Strand
+ * s = 2r (forward) or 2r+1 (reverse complement): up to max_hits best hits
+ * at [s * max_hits ..] of ref_idx_d/shift_d/ref_pos_d, their count in
+ * nbest_d[s], the number of clusters tied at the biggest size in
+ * nbiggest_d[s] and the k-mer hits gathered in nhits_d[s] (nbiggest_d and
+ * nhits_d may be NULL).  Synchronises the stream before returning. */
+int swift_map_reads(const swift_index *ix, const char *reads_d, const int64_t *off_d,
+                    const int32_t *len_d, int nreads, int max_read_len, int max_hits,
+                    int8_t *ref_idx_d, int32_t *shift_d, int32_t *ref_pos_d, int32_t *nbest_d,
+                    int32_t *nbiggest_d, int32_t *nhits_d, void *stream);
+
+/* Phase 1 -> phase 2 on the device: records (one per best hit, read order,
+ * forward strand first, hits in cluster order) with the region window, the
+ * Smith-Waterman result against it and the per-read best-hit choice.  All
+ * pointers are device memory with room for `capacity` records
+ * (>= 2 * nreads * max_hits); cons_stride >= max_read_len + 302. */
+typedef struct {
+    int32_t *nrecords;        /* [1] records written */
+    int32_t *read, *strand, *ref_idx, *shift, *ref_pos, *nbiggest;
+    int64_t *ref_distance;    /* window start: max(0, refPos - 100) */
+    int32_t *score, *start, *end, *alen;
+    int32_t *keep;            /* 1 for the record chooseBestAlignment keeps */
+    int32_t *ntied;           /* records sharing the read's best score */
+    char *qry_cons, *ref_cons; /* record i at i * cons_stride, walk order */
+    int64_t cons_stride;
+    int32_t capacity;
+} swift_records;
+
+/* Reads up to 320 bases; windows of 300 bases from max(0, refPos - 100)
+ * (alignQueries_cpu_mt, src/align.cu:2363-2511; chooseBestAlignment,
+ * src/align.cu:844-905).  Synchronises the stream before returning. */
+int swift_align_reads(const swift_index *ix, const char *reads_d, const int64_t *off_d,
+                      const int32_t *len_d, int nreads, int max_read_len, int max_hits,
+                      int match, int mismatch, int gap_open, int gap_ext,
+                      const swift_records *rec, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

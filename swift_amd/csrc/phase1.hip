@@ -1,0 +1,718 @@
+// phase1.hip -- k-mer index construction, read mapping, region extraction
+// and best-hit choice on gfx950, with the semantics of the reference CPU
+// path:
+//   lookupTable5Create (src/lookupTable5.cu:101-252): non-overlapping
+//     k-mers at positions 0, k, 2k, ... of each sequence, hash
+//     sum_i 4^i code(s[i]) (getHash, src/preprocess.cu:70-91), a hash seen
+//     more than thr times dropped entirely, buckets in hash order starting
+//     at 1, entries in reverse file order within a bucket;
+//   lookupTable5MapQry + mapHits6GetBestHits (src/lookupTable5.cu:557-603,
+//     src/mapHits6.cu:40-167): every overlapping query k-mer gathers its
+//     bucket as hits (refIdx, refPos - i, refPos); clusters are runs of equal
+//     (refIdx, shift) after sorting by (refIdx, shift, refPos); the first hit
+//     of each biggest cluster is reported, at most maxHits of them, in sorted
+//     order.  The reference draws among more than maxHits tied clusters with
+//     rand() (src/mapHits6.cu:146-160); this build keeps the first maxHits,
+//     one member of the reference's tie set, and reports the set's size.
+//
+// Layout: the index lives in HBM as an int2 bucket table (start, kept count)
+// plus 64-bit entries (refIdx << 32 | refPos).  One wave maps one read
+// strand: lanes hash query offsets, a wave prefix sum lays the hits out, the
+// wave gathers bucket entries cooperatively (one hit per lane), sorts 64-bit
+// hit keys in LDS and compacts the biggest clusters with ballots.  Strands
+// whose hits exceed LDS take the same steps in a global scratch slot with a
+// 256-thread workgroup.
+#include <algorithm>
+
+#include "phase1.hpp"
+
+namespace swift {
+
+__device__ __forceinline__ uint32_t kcode(unsigned char c) {
+    // preprocessCreate (src/preprocess.cu:30-48): A/a 0, C/c 1, G/g 2,
+    // T/t 3, every other byte 0
+    switch (c) {
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 0;
+    }
+}
+
+__device__ __forceinline__ unsigned char rc_byte(unsigned char c) {
+    // qryGetReverseComplement (src/query.cu:265-282): anything but A/C/G -> A
+    return c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
+}
+
+// first base = least significant base-4 digit
+__device__ __forceinline__ uint32_t kmer_hash(const unsigned char *p, int k) {
+    uint32_t h = 0;
+    for (int i = k - 1; i >= 0; --i) h = h * 4 + kcode(p[i]);
+    return h;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Bitonic sort of n2 (a power of two) keys by threads tid, tid + nthr, ...;
+// `sync` orders the stages.  DESC = false sorts ascending.
+template <bool DESC, class K, class Sync>
+__device__ void bitonic(K *key, int n2, int tid, int nthr, Sync sync) {
+    for (int kk = 2; kk <= n2; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < n2; i += nthr) {
+                int p = i ^ j;
+                if (p > i) {
+                    bool asc = ((i & kk) == 0) != DESC;
+                    K x = key[i], y = key[p];
+                    if (asc ? (y < x) : (x < y)) {
+                        key[i] = y;
+                        key[p] = x;
+                    }
+                }
+            }
+            sync();
+        }
+    }
+}
+
+__device__ __forceinline__ int next_pow2(int n) {
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    return n2;
+}
+
+// ---------------------------------------------------------------------------
+// Index build
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int seq_of_tuple(const int64_t *tup0, int nseq, int64_t t) {
+    int lo = 0, hi = nseq - 1;  // last s with tup0[s] <= t
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tup0[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) idx_count(IndexDev ix, int64_t ntup) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntup;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int s = seq_of_tuple(ix.tup0, ix.nseq, t);
+        int64_t pos = (t - ix.tup0[s]) * ix.k;
+        uint32_t h = kmer_hash((const unsigned char *)ix.seq + ix.seq_off[s] + pos, ix.k);
+        atomicAdd(&ix.count[h], 1);
+    }
+}
+
+__device__ __forceinline__ int kept(int c, int thr) { return (c > 0 && c <= thr) ? c : 0; }
+
+// per-chunk sums of the kept counts and the largest kept count
+__global__ void __launch_bounds__(1024) idx_scan_blocks(IndexDev ix, IndexBuildScratch s) {
+    __shared__ int64_t red[1024];
+    __shared__ int32_t redm[1024];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    int64_t sum = 0;
+    int32_t mx = 0;
+    for (int i = threadIdx.x; i < kScanChunk; i += 1024) {
+        int64_t h = base + i;
+        if (h < ix.nhash) {
+            int c = kept(ix.count[h], ix.thr);
+            sum += c;
+            mx = max(mx, c);
+        }
+    }
+    red[threadIdx.x] = sum;
+    redm[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            red[threadIdx.x] += red[threadIdx.x + off];
+            redm[threadIdx.x] = max(redm[threadIdx.x], redm[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s.block_sum[blockIdx.x] = red[0];
+        atomicMax(s.max_actual, redm[0]);
+    }
+}
+
+// exclusive scan of the chunk sums (one workgroup); the total into *total
+__global__ void __launch_bounds__(1024) idx_scan_sums(IndexBuildScratch s, int nblocks) {
+    __shared__ int64_t part[1024];
+    const int per = (nblocks + 1023) / 1024;
+    int64_t sum = 0;
+    for (int i = 0; i < per; ++i) {
+        int b = threadIdx.x * per + i;
+        if (b < nblocks) sum += s.block_sum[b];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int64_t run = part[threadIdx.x] - sum;
+    for (int i = 0; i < per; ++i) {
+        int b = threadIdx.x * per + i;
+        if (b < nblocks) {
+            int64_t v = s.block_sum[b];
+            s.block_sum[b] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 1023) *s.total = part[1023];
+}
+
+// start = 1 + exclusive prefix of the kept counts (0 for empty or dropped
+// hashes), actual = kept count (src/lookupTable5.cu:199-208)
+__global__ void __launch_bounds__(1024) idx_scan_apply(IndexDev ix, IndexBuildScratch s) {
+    __shared__ int64_t part[1024];
+    constexpr int per = kScanChunk / 1024;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * per;
+    int32_t loc[per];
+    int64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < per; ++i) {
+        int64_t h = base + i;
+        loc[i] = h < ix.nhash ? kept(ix.count[h], ix.thr) : 0;
+        sum += loc[i];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int64_t run = s.block_sum[blockIdx.x] + part[threadIdx.x] - sum;
+#pragma unroll
+    for (int i = 0; i < per; ++i) {
+        int64_t h = base + i;
+        if (h < ix.nhash) {
+            int32_t st = loc[i] ? (int32_t)(run + 1) : 0;
+            ix.start[h] = st;
+            ix.actual[h] = loc[i];
+            ix.bucket[h] = make_int2(st, loc[i]);
+            ix.cursor[h] = 0;
+        }
+        run += loc[i];
+    }
+}
+
+// scatter every kept tuple into its bucket (order fixed by the sort below)
+__global__ void __launch_bounds__(256) idx_fill(IndexDev ix, int64_t ntup) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntup;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int s = seq_of_tuple(ix.tup0, ix.nseq, t);
+        int64_t pos = (t - ix.tup0[s]) * ix.k;
+        uint32_t h = kmer_hash((const unsigned char *)ix.seq + ix.seq_off[s] + pos, ix.k);
+        if (ix.actual[h] == 0) continue;
+        int slot = ix.start[h] + atomicAdd(&ix.cursor[h], 1);
+        ix.entry[slot] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)pos;
+    }
+}
+
+// Reverse file order within a bucket (src/lookupTable5.cu:219-229) is
+// descending (refIdx, refPos).  Buckets of <= kSmallBucket entries are sorted
+// by one thread in registers; larger ones are listed for the wave sort or
+// the workgroup sort.
+__global__ void __launch_bounds__(256) idx_sort_small(IndexDev ix, IndexBuildScratch s) {
+    int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (h >= ix.nhash) return;
+    const int a = ix.actual[h];
+    if (a < 2) return;
+    if (a > kSmallBucket) {
+        if (a <= kWaveBucket) s.mid_list[atomicAdd(s.mid_count, 1)] = (int32_t)h;
+        else s.big_list[atomicAdd(s.big_count, 1)] = (int32_t)h;
+        return;
+    }
+    uint64_t *e = ix.entry + ix.start[h];
+    uint64_t v[kSmallBucket];
+#pragma unroll
+    for (int i = 0; i < kSmallBucket; ++i) v[i] = i < a ? e[i] : 0;
+    // padding zeros sort last; a real entry equal to 0 is indistinguishable
+#pragma unroll
+    for (int i = 1; i < kSmallBucket; ++i) {
+#pragma unroll
+        for (int j = i; j > 0; --j) {
+            uint64_t x = v[j - 1], y = v[j];
+            bool sw = y > x;
+            v[j - 1] = sw ? y : x;
+            v[j] = sw ? x : y;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSmallBucket; ++i)
+        if (i < a) e[i] = v[i];
+}
+
+// one wave per listed bucket, keys in LDS
+__global__ void __launch_bounds__(64) idx_sort_mid(IndexDev ix, IndexBuildScratch s) {
+    __shared__ uint64_t key[kWaveBucket];
+    const int n = *s.mid_count;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int h = s.mid_list[it];
+        const int a = ix.actual[h];
+        uint64_t *e = ix.entry + ix.start[h];
+        const int n2 = next_pow2(a);
+        for (int i = threadIdx.x; i < n2; i += 64) key[i] = i < a ? e[i] : 0;
+        __syncthreads();
+        bitonic<true>(key, n2, threadIdx.x, 64, [] { __syncthreads(); });
+        for (int i = threadIdx.x; i < a; i += 64) e[i] = key[i];
+        __syncthreads();
+    }
+}
+
+// one workgroup per listed bucket, keys in a global scratch slot
+__global__ void __launch_bounds__(256) idx_sort_big(IndexDev ix, IndexBuildScratch s) {
+    const int n = *s.big_count;
+    uint64_t *key = s.big_scratch + (int64_t)blockIdx.x * s.big_cap;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int h = s.big_list[it];
+        const int a = ix.actual[h];
+        uint64_t *e = ix.entry + ix.start[h];
+        const int n2 = next_pow2(a);
+        for (int i = threadIdx.x; i < n2; i += 256) key[i] = i < a ? e[i] : 0;
+        __syncthreads();
+        bitonic<true>(key, n2, threadIdx.x, 256, [] { __syncthreads(); });
+        for (int i = threadIdx.x; i < a; i += 256) e[i] = key[i];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mapper
+// ---------------------------------------------------------------------------
+// hit key: refIdx (bits 43..49) | shift + kShiftBias (bits 11..42) | query
+// offset (bits 0..10); ascending keys = the reference's (refIdx, shift,
+// refPos) order because refPos = shift + offset
+__device__ __forceinline__ uint64_t hit_key(uint64_t entry, int i) {
+    uint64_t ref = entry >> 32;
+    uint64_t shift_b = (uint64_t)((int64_t)(uint32_t)entry - i + kShiftBias);
+    return (ref << 43) | (shift_b << 11) | (uint64_t)i;
+}
+
+// Cluster compaction over sorted keys[0..n) by one wave: the biggest run
+// size, the number of runs of that size, and the first `max_hits` of them
+// in sorted order (findBiggestClusterSize / findBiggestClusters,
+// src/mapHits6.cu:202-223,132-167).
+template <class K, class ClusterOf, class Emit>
+__device__ void wave_best_clusters(const K *key, int n, int max_hits, ClusterOf cl, Emit emit,
+                                   int32_t *o_nbest, int32_t *o_nbig) {
+    const int lane = threadIdx.x & 63;
+    auto run_at = [&](int e) {
+        if (e >= n || (e > 0 && cl(key[e]) == cl(key[e - 1]))) return 0;
+        int len = 1;
+        const auto c = cl(key[e]);
+        while (e + len < n && cl(key[e + len]) == c) ++len;
+        return len;
+    };
+    int best = 0;
+    for (int b = 0; b < n; b += 64) best = max(best, run_at(b + lane));
+#pragma unroll
+    for (int sft = 32; sft >= 1; sft >>= 1) best = max(best, __shfl_xor(best, sft, 64));
+    int found = 0;
+    for (int b = 0; b < n; b += 64) {
+        const int e = b + lane;
+        const bool big = best > 0 && run_at(e) == best;
+        const uint64_t mask = __ballot(big);
+        const int slot = found + __popcll(mask & ((1ull << lane) - 1));
+        if (big && slot < max_hits) emit(slot, key[e]);
+        found += __popcll(mask);
+    }
+    if (lane == 0) {
+        *o_nbest = min(found, max_hits);
+        *o_nbig = found;
+    }
+}
+
+struct HitCluster {
+    __device__ uint64_t operator()(uint64_t k) const { return k >> 11; }
+};
+
+// Gathers the hits of one strand into kbuf (capacity cap) with one wave.
+// Returns the hit count; *ok is false when it exceeds cap (kbuf then holds
+// a prefix).  ex/stt: 64-int LDS arrays.
+__device__ int gather_hits(const IndexDev &ix, const unsigned char *qb, int nq, uint64_t *kbuf,
+                           int64_t cap, int32_t *ex, int32_t *stt, bool *ok) {
+    const int lane = threadIdx.x & 63;
+    int total = 0;
+    bool fits = true;
+    for (int b = 0; b < nq; b += 64) {
+        const int i = b + lane;
+        int cnt = 0, st = 0;
+        if (i < nq) {
+            int2 bk = ix.bucket[kmer_hash(qb + i, ix.k)];
+            st = bk.x;
+            cnt = bk.y;
+        }
+        const int incl = wave_incl_scan(cnt);
+        const int tot = __shfl(incl, 63, 64);
+        if (fits && (int64_t)total + tot <= cap) {
+            ex[lane] = incl - cnt;
+            stt[lane] = st;
+            wave_sync();
+            for (int e = lane; e < tot; e += 64) {
+                int o = 0;  // last lane whose exclusive offset is <= e
+#pragma unroll
+                for (int sft = 32; sft >= 1; sft >>= 1)
+                    if (ex[o + sft] <= e) o += sft;
+                kbuf[total + e] = hit_key(ix.entry[stt[o] + (e - ex[o])], b + o);
+            }
+            wave_sync();
+        } else {
+            fits = false;
+        }
+        total += tot;
+    }
+    *ok = fits;
+    return total;
+}
+
+__device__ void load_strand(const MapArgs &a, int sidx, unsigned char *qb, int *len_out) {
+    const int r = sidx >> 1, strand = sidx & 1;
+    int L = a.rd.len[r];
+    if (L < 0 || L > a.max_len) {
+        if (threadIdx.x == 0) atomicOr(a.error, 2);
+        L = 0;
+    }
+    const unsigned char *src = (const unsigned char *)a.rd.seq + a.rd.off[r];
+    char *dst = a.strand_seq ? a.strand_seq + (int64_t)sidx * a.qstride : nullptr;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        unsigned char c = src[i];
+        int at = strand ? L - 1 - i : i;
+        unsigned char v = strand ? rc_byte(c) : c;
+        qb[at] = v;
+        if (dst) dst[at] = (char)v;
+    }
+    *len_out = L;
+}
+
+__device__ void emit_best(const MapArgs &a, int sidx, const uint64_t *keys, int n) {
+    const int64_t o = (int64_t)sidx * a.out.stride;
+    wave_best_clusters(
+        keys, n, a.max_hits, HitCluster{},
+        [&](int slot, uint64_t k) {
+            int sh = (int)((int64_t)((k >> 11) & 0xFFFFFFFFull) - kShiftBias);
+            a.out.ref[o + slot] = (int8_t)(k >> 43);
+            a.out.shift[o + slot] = sh;
+            a.out.pos[o + slot] = sh + (int)(k & 2047);
+        },
+        a.out.nbest + sidx, a.out.nbig + sidx);
+}
+
+// one wave per strand; strands whose hits overflow LDS are listed
+__global__ void __launch_bounds__(64) map_strands(MapArgs a) {
+    __shared__ uint64_t keys[kMapCap];
+    __shared__ unsigned char qb[kMaxMapRead + 1];
+    __shared__ int32_t ex[64], stt[64];
+    const int sidx = blockIdx.x;
+    const int lane = threadIdx.x;
+    int L;
+    load_strand(a, sidx, qb, &L);
+    __syncthreads();
+    bool ok;
+    const int total = gather_hits(a.ix, qb, L - a.ix.k + 1, keys, kMapCap, ex, stt, &ok);
+    if (lane == 0) a.out.nhits[sidx] = total;
+    if (total == 0) {
+        if (lane == 0) {
+            a.out.nbest[sidx] = 0;
+            a.out.nbig[sidx] = 0;
+        }
+        return;
+    }
+    if (!ok) {
+        if (lane == 0) a.ovl_list[atomicAdd(a.ovl_count, 1)] = sidx;
+        return;
+    }
+    const int n2 = next_pow2(total);
+    for (int i = total + lane; i < n2; i += 64) keys[i] = ~0ull;
+    __syncthreads();
+    bitonic<false>(keys, n2, lane, 64, [] { __syncthreads(); });
+    emit_best(a, sidx, keys, total);
+}
+
+// listed strands: one 256-thread workgroup each, keys in a scratch slot
+__global__ void __launch_bounds__(256) map_strands_big(MapArgs a) {
+    __shared__ unsigned char qb[kMaxMapRead + 1];
+    __shared__ int32_t ex[64], stt[64];
+    __shared__ int32_t sh_total, sh_ok;
+    const int n = *a.ovl_count;
+    uint64_t *kbuf = a.scratch + (int64_t)blockIdx.x * a.scratch_cap;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int sidx = a.ovl_list[it];
+        int L;
+        load_strand(a, sidx, qb, &L);
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            bool ok;
+            int total = gather_hits(a.ix, qb, L - a.ix.k + 1, kbuf, a.scratch_cap, ex, stt, &ok);
+            if (threadIdx.x == 0) {
+                sh_total = total;
+                sh_ok = ok;
+            }
+        }
+        __syncthreads();
+        const int total = sh_total;
+        if (!sh_ok) {  // beyond the scratch slot: reported as a capacity error
+            if (threadIdx.x == 0) {
+                atomicOr(a.error, 1);
+                a.out.nbest[sidx] = 0;
+                a.out.nbig[sidx] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        const int n2 = next_pow2(total);
+        for (int i = total + threadIdx.x; i < n2; i += 256) kbuf[i] = ~0ull;
+        __syncthreads();
+        bitonic<false>(kbuf, n2, threadIdx.x, 256, [] { __syncthreads(); });
+        if (threadIdx.x < 64) emit_best(a, sidx, kbuf, total);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Records: per-strand prefix, hit -> record, region window per record
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) strand_prefix(const int32_t *nbest, int nstrands,
+                                                      int32_t *pair0, int32_t *npairs) {
+    __shared__ int32_t part[1024];
+    const int per = (nstrands + 1023) / 1024;
+    int32_t sum = 0;
+    for (int i = 0; i < per; ++i) {
+        int t = threadIdx.x * per + i;
+        if (t < nstrands) sum += nbest[t];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int32_t run = part[threadIdx.x] - sum;
+    for (int i = 0; i < per; ++i) {
+        int t = threadIdx.x * per + i;
+        if (t < nstrands) {
+            pair0[t] = run;
+            run += nbest[t];
+        }
+    }
+    if (threadIdx.x == 1023) {
+        pair0[nstrands] = part[1023];
+        *npairs = part[1023];
+    }
+}
+
+__global__ void __launch_bounds__(256) hits_to_records(MapArgs m, PairsDev p) {
+    const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sidx >= 2 * m.rd.nreads) return;
+    const int nb = m.out.nbest[sidx];
+    const int at = p.pair0[sidx];
+    const int64_t o = (int64_t)sidx * m.out.stride;
+    for (int h = 0; h < nb; ++h) {
+        p.read[at + h] = sidx >> 1;
+        p.strand[at + h] = sidx & 1;
+        p.ref[at + h] = m.out.ref[o + h];
+        p.shift[at + h] = m.out.shift[o + h];
+        p.pos[at + h] = m.out.pos[o + h];
+        p.nbig[at + h] = m.out.nbig[sidx];
+    }
+}
+
+__device__ __forceinline__ bool is_n(char c) { return c == 'N' || c == 'n'; }
+
+// Window of record i: refDistance = max(0, refPos - before), up to win
+// bases clipped at the sequence end (alignQueries_cpu_mt, src/align.cu
+// :2388,2435); leading N/n stripped, trailing N/n stripped only when no
+// leading strip happened, refDistance left unchanged (getRefSeq2,
+// src/input.cu:348-403).  Slots past the record count get empty pairs.
+__global__ void __launch_bounds__(256) build_regions(MapArgs m, PairsDev p, int before, int win) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.capacity) return;
+    p.coff[i] = (int64_t)i * p.cstride;
+    if (i >= *p.npairs) {
+        p.qlen[i] = 0;
+        p.rlen[i] = 0;
+        p.qoff[i] = 0;
+        p.roff[i] = 0;
+        return;
+    }
+    const int s = p.ref[i];
+    const int64_t slen = m.ix.seq_len[s];
+    int64_t rd = (int64_t)p.pos[i] - before;
+    if (rd < 0) rd = 0;
+    int64_t n = slen - rd;
+    if (n > win) n = win;
+    if (n < 0) n = 0;
+    const char *w = m.ix.seq + m.ix.seq_off[s] + rd;
+    int64_t lead = 0;
+    while (lead < n && is_n(w[lead])) ++lead;
+    if (lead == 0)
+        while (n > 0 && is_n(w[n - 1])) --n;
+    const int sidx = 2 * p.read[i] + p.strand[i];
+    p.rdist[i] = rd;
+    p.roff[i] = m.ix.seq_off[s] + rd + lead;
+    p.rlen[i] = (int32_t)(n - lead);
+    p.qoff[i] = (int64_t)sidx * m.qstride;
+    p.qlen[i] = m.rd.len[p.read[i]];
+}
+
+// chooseBestAlignment (src/align.cu:844-905) per read: the maximum starts at
+// 0 and rises on a strict <, the records equal to it form the tie set, the
+// first of them is kept (the reference keeps the rand() % j-th).
+__global__ void __launch_bounds__(256) choose_best(PairsDev p, int nreads, const int32_t *score,
+                                                   int32_t *keep, int32_t *ntied) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreads) return;
+    const int a = p.pair0[2 * r], b = p.pair0[2 * r + 2];
+    int mx = 0, j = 0;
+    for (int i = a; i < b; ++i) mx = max(mx, score[i]);
+    for (int i = a; i < b; ++i) j += score[i] == mx;
+    bool done = false;
+    for (int i = a; i < b; ++i) {
+        bool k = !done && score[i] == mx;
+        keep[i] = k;
+        done |= k;
+        ntied[i] = j;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mapHits6GetBestHits over caller hits (drop-in; one workgroup)
+// ---------------------------------------------------------------------------
+struct HitTriple {
+    uint64_t hi;  // (refIdx + 128) << 32 | (shift ^ 0x80000000)
+    uint32_t lo;  // refPos ^ 0x80000000
+    uint32_t pad;
+    __device__ bool operator<(const HitTriple &o) const {
+        return hi < o.hi || (hi == o.hi && lo < o.lo);
+    }
+};
+struct TripleCluster {
+    __device__ uint64_t operator()(const HitTriple &t) const { return t.hi; }
+};
+
+__global__ void __launch_bounds__(256)
+best_hits_kernel(int8_t *ref, int32_t *shift, int32_t *pos, int n, int max_hits, int8_t *o_ref,
+                 int32_t *o_shift, int32_t *o_pos, int32_t *o_n, int32_t *o_nbig,
+                 HitTriple *key) {
+    const int n2 = next_pow2(n);
+    for (int i = threadIdx.x; i < n2; i += 256) {
+        HitTriple t;
+        if (i < n) {
+            t.hi = ((uint64_t)(uint32_t)(ref[i] + 128) << 32) | ((uint32_t)shift[i] ^ 0x80000000u);
+            t.lo = (uint32_t)pos[i] ^ 0x80000000u;
+        } else {
+            t.hi = ~0ull;
+            t.lo = ~0u;
+        }
+        t.pad = 0;
+        key[i] = t;
+    }
+    __syncthreads();
+    bitonic<false>(key, n2, threadIdx.x, 256, [] { __syncthreads(); });
+    for (int i = threadIdx.x; i < n; i += 256) {
+        HitTriple t = key[i];
+        ref[i] = (int8_t)((int)(t.hi >> 32) - 128);
+        shift[i] = (int32_t)((uint32_t)t.hi ^ 0x80000000u);
+        pos[i] = (int32_t)(t.lo ^ 0x80000000u);
+    }
+    if (threadIdx.x < 64)
+        wave_best_clusters(
+            key, n, max_hits, TripleCluster{},
+            [&](int slot, const HitTriple &t) {
+                o_ref[slot] = (int8_t)((int)(t.hi >> 32) - 128);
+                o_shift[slot] = (int32_t)((uint32_t)t.hi ^ 0x80000000u);
+                o_pos[slot] = (int32_t)(t.lo ^ 0x80000000u);
+            },
+            o_n, o_nbig);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+static int tuple_grid(int64_t ntup) {
+    int64_t g = (ntup + 255) / 256;
+    return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 1 << 16);
+}
+
+hipError_t launch_index_build(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,
+                              hipStream_t st) {
+    if (ntup > 0) hipLaunchKernelGGL(idx_count, dim3(tuple_grid(ntup)), dim3(256), 0, st, ix, ntup);
+    const int nblocks = (int)((ix.nhash + kScanChunk - 1) / kScanChunk);
+    hipLaunchKernelGGL(idx_scan_blocks, dim3(nblocks), dim3(1024), 0, st, ix, s);
+    hipLaunchKernelGGL(idx_scan_sums, dim3(1), dim3(1024), 0, st, s, nblocks);
+    hipLaunchKernelGGL(idx_scan_apply, dim3(nblocks), dim3(1024), 0, st, ix, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,
+                             hipStream_t st) {
+    if (ntup > 0) hipLaunchKernelGGL(idx_fill, dim3(tuple_grid(ntup)), dim3(256), 0, st, ix, ntup);
+    hipLaunchKernelGGL(idx_sort_small, dim3((unsigned)((ix.nhash + 255) / 256)), dim3(256), 0, st,
+                       ix, s);
+    hipLaunchKernelGGL(idx_sort_mid, dim3(8192), dim3(64), 0, st, ix, s);
+    if (s.big_slots > 0)
+        hipLaunchKernelGGL(idx_sort_big, dim3(s.big_slots), dim3(256), 0, st, ix, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_map(const MapArgs &a, hipStream_t st) {
+    if (a.rd.nreads <= 0) return hipSuccess;
+    hipLaunchKernelGGL(map_strands, dim3(2 * a.rd.nreads), dim3(64), 0, st, a);
+    if (a.scratch_slots > 0)
+        hipLaunchKernelGGL(map_strands_big, dim3(a.scratch_slots), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_records(const MapArgs &m, const PairsDev &p, int before, int win,
+                          hipStream_t st) {
+    const int ns = 2 * m.rd.nreads;
+    hipLaunchKernelGGL(strand_prefix, dim3(1), dim3(1024), 0, st, m.out.nbest, ns, p.pair0,
+                       p.npairs);
+    if (ns > 0)
+        hipLaunchKernelGGL(hits_to_records, dim3((ns + 255) / 256), dim3(256), 0, st, m, p);
+    if (p.capacity > 0)
+        hipLaunchKernelGGL(build_regions, dim3((p.capacity + 255) / 256), dim3(256), 0, st, m, p,
+                           before, win);
+    return hipGetLastError();
+}
+
+hipError_t launch_choose_best(const PairsDev &p, int nreads, const int32_t *score, int32_t *keep,
+                              int32_t *ntied, hipStream_t st) {
+    if (nreads > 0)
+        hipLaunchKernelGGL(choose_best, dim3((nreads + 255) / 256), dim3(256), 0, st, p, nreads,
+                           score, keep, ntied);
+    return hipGetLastError();
+}
+
+hipError_t launch_best_hits(int8_t *ref, int32_t *shift, int32_t *pos, int n, int max_hits,
+                            int8_t *o_ref, int32_t *o_shift, int32_t *o_pos, int32_t *o_n,
+                            int32_t *o_nbig, void *kbuf, hipStream_t st) {
+    hipLaunchKernelGGL(best_hits_kernel, dim3(1), dim3(256), 0, st, ref, shift, pos, n, max_hits,
+                       o_ref, o_shift, o_pos, o_n, o_nbig, static_cast<HitTriple *>(kbuf));
+    return hipGetLastError();
+}
+
+}  // namespace swift

@@ -1,0 +1,112 @@
+// phase1.hpp -- device-side data layout of phase 1 (k-mer index, read
+// mapping, region extraction, best-hit choice) shared by phase1.hip and the
+// host code.
+#pragma once
+
+#include "swift_common.hpp"
+
+namespace swift {
+
+// Limits of the device mapper.  Hit keys pack (refIdx, shift, offset) into
+// 64 bits: refIdx < 2^7 (the reference stores it in a signed char),
+// shift + kShiftBias < 2^32, query offset < 2^11.
+constexpr int kMaxSeeds = 15;          // k <= 15 keeps 4^k buckets < 2^30
+constexpr int kMaxIndexSeqs = 127;     // refIdx is a signed char in the reference
+constexpr int kMaxMapRead = 2047;      // query offsets fit 11 bits
+constexpr int64_t kShiftBias = 2048;   // shift >= -(kMaxMapRead - 1)
+constexpr int kMapCap = 1024;          // hits per strand sorted in LDS
+constexpr int kSmallBucket = 32;       // buckets sorted by one thread
+constexpr int kWaveBucket = 1024;      // buckets sorted by one wave in LDS
+constexpr int kScanChunk = 4096;       // hashes per scan workgroup
+
+// Index arrays in HBM.  Sequences are stored as the index sees them (the
+// reference's line handling applied, see the FASTA reader), back to back.
+struct IndexDev {
+    const char *seq;          // bases of every sequence
+    const int64_t *seq_off;   // [nseq] offset of sequence s in seq
+    const int64_t *seq_len;   // [nseq]
+    const int64_t *tup0;      // [nseq] first tuple number of sequence s
+    int nseq, k, thr;
+    int64_t nhash;            // 4^k
+    int32_t *count;           // [nhash] occurrences (_numRepeatsPerTuple)
+    int32_t *start;           // [nhash] 1-based bucket start, 0 if empty/dropped
+    int32_t *actual;          // [nhash] kept occurrences (0 if count > thr)
+    int32_t *cursor;          // [nhash] fill cursor
+    int2 *bucket;             // [nhash] (start, actual) for the mapper
+    uint64_t *entry;          // [total + 1] refIdx << 32 | refPos, [0] sentinel
+};
+
+struct IndexBuildScratch {
+    int64_t *block_sum;       // [nhash / kScanChunk + 1]
+    int64_t *total;           // [1] kept tuples
+    int32_t *max_actual;      // [1]
+    int32_t *mid_list, *mid_count;   // buckets for the wave sort
+    int32_t *big_list, *big_count;   // buckets for the block sort
+    uint64_t *big_scratch;    // [big_slots * big_cap]
+    int32_t big_slots, big_cap;
+};
+
+struct ReadsDev {
+    const char *seq;
+    const int64_t *off;
+    const int32_t *len;
+    int nreads;
+};
+
+// Per strand s = 2 * read + strand (strand 1 = reverse complement): up to
+// `stride` best hits and the counters of the mapping.
+struct MapOut {
+    int8_t *ref;
+    int32_t *shift, *pos;
+    int32_t *nbest;           // hits reported (<= max_hits)
+    int32_t *nbig;            // clusters of the biggest size (tie-set size)
+    int32_t *nhits;           // k-mer hits gathered
+    int stride;
+};
+
+struct MapArgs {
+    IndexDev ix;
+    ReadsDev rd;
+    MapOut out;
+    int max_hits;
+    int max_len;              // longer reads (or negative lengths) map as empty
+    char *strand_seq;         // optional: strand bases at s * qstride (>= max_len)
+    int qstride;
+    uint64_t *scratch;        // overflow strands: slots of scratch_cap keys
+    int64_t scratch_cap;
+    int scratch_slots;
+    int32_t *ovl_list, *ovl_count;
+    int32_t *error;           // bit 0: a strand exceeded scratch_cap, bit 1: a read
+                              // longer than max_len
+};
+
+// Records of the phase-1 -> phase-2 pipeline, one per reported hit, in read
+// order, forward strand first, hits in cluster order.
+struct PairsDev {
+    int32_t *npairs;          // [1]
+    int32_t *pair0;           // [2 * nreads + 1] first record of each strand
+    int32_t *read, *strand, *ref, *shift, *pos, *nbig;
+    int64_t *rdist;           // window start in the sequence
+    int64_t *qoff, *roff, *coff;
+    int32_t *qlen, *rlen;
+    int capacity;
+    int64_t cstride;
+};
+
+hipError_t launch_index_build(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,
+                              hipStream_t st);
+hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,
+                             hipStream_t st);
+hipError_t launch_map(const MapArgs &a, hipStream_t st);
+hipError_t launch_records(const MapArgs &m, const PairsDev &p, int before, int win,
+                          hipStream_t st);
+hipError_t launch_choose_best(const PairsDev &p, int nreads, const int32_t *score,
+                              int32_t *keep, int32_t *ntied, hipStream_t st);
+// mapHits6GetBestHits over caller hits already on the device: sorts the
+// triples in place (refIdx, shift, refPos ascending) and writes the best
+// clusters' first hits.  kbuf holds next_pow2(n) 16-byte keys.
+hipError_t launch_best_hits(int8_t *ref, int32_t *shift, int32_t *pos, int n, int max_hits,
+                            int8_t *o_ref, int32_t *o_shift, int32_t *o_pos, int32_t *o_n,
+                            int32_t *o_nbig, void *kbuf, hipStream_t st);
+
+}  // namespace swift

@@ -1,0 +1,825 @@
+// phase1_host.cpp -- host side of phase 1: FASTA readers with the
+// reference's line handling, the device-resident index, read mapping, the
+// phase-1 -> phase-2 pipeline, and the reference's phase-1 entry points
+// (lookupTable5*, mapHits6GetBestHits, getHash, qryGetReverseComplement,
+// searchQueries3) on top of them.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/swift_abi.h"
+#include "phase1.hpp"
+#include "sw_host.hpp"
+#include "sw_types.hpp"
+
+namespace swift {
+
+// Exact-size device array, freed with its owner.
+struct DevArr {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int alloc(size_t b) {
+        release();
+        SWIFT_HIP_TRY(hipMalloc(&ptr, b ? b : 1));
+        bytes = b;
+        return kOk;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+    DevArr() = default;
+    DevArr(const DevArr &) = delete;
+    DevArr &operator=(const DevArr &) = delete;
+    ~DevArr() { release(); }
+};
+
+// ---------------------------------------------------------------------------
+// FASTA readers
+// ---------------------------------------------------------------------------
+struct RefSeqs {
+    std::vector<char> bases;
+    std::vector<int64_t> off, len;
+    std::vector<std::string> names;
+};
+
+static bool slurp(const char *path, std::vector<char> *buf) {
+    FILE *f = fopen(path, "rb");
+    if (!f) return false;
+    buf->clear();
+    char tmp[1 << 16];
+    size_t n;
+    while ((n = fread(tmp, 1, sizeof(tmp), f)) > 0) buf->insert(buf->end(), tmp, tmp + n);
+    fclose(f);
+    return true;
+}
+
+// lookupTable5Create's reading of the reference (src/lookupTable5.cu:130-185):
+// a '>' line starts the next sequence; the bases of the following lines are
+// concatenated (tuples run across line breaks through the carried tail, and
+// a blank line keeps that tail, because the line length the reference tests
+// includes it).  The sequence as indexed is therefore the concatenation of
+// its lines without the newline bytes.
+static int read_reference_fasta(const char *path, RefSeqs *out) {
+    std::vector<char> buf;
+    if (!slurp(path, &buf)) {
+        set_error("cannot read reference file %s", path);
+        return kErrIO;
+    }
+    out->bases.clear();
+    out->off.clear();
+    out->len.clear();
+    out->names.clear();
+    size_t i = 0;
+    while (i < buf.size()) {
+        size_t e = i;
+        while (e < buf.size() && buf[e] != '\n') ++e;
+        if (e > i) {
+            if (buf[i] == '>') {
+                out->names.emplace_back(buf.data() + i + 1, e - i - 1);
+                out->off.push_back((int64_t)out->bases.size());
+                out->len.push_back(0);
+            } else {
+                if (out->off.empty()) {
+                    set_error("%s: sequence data before the first '>' header", path);
+                    return kErrInvalid;
+                }
+                out->bases.insert(out->bases.end(), buf.begin() + i, buf.begin() + e);
+                out->len.back() += (int64_t)(e - i);
+            }
+        }
+        i = e + 1;
+    }
+    return kOk;
+}
+
+// qryGetNext2 (src/query.cu:174-214): a '>' line names the next read and
+// sets its offsets (nameOffset = start of the header line, seqOffset = the
+// byte after it); each sequence line is one read.  Blank lines are skipped.
+struct QueryFile {
+    std::vector<char> bases;
+    std::vector<int64_t> off;
+    std::vector<int32_t> len;
+    std::vector<std::string> names;
+    std::vector<unsigned long> name_off, seq_off;
+};
+
+static int read_query_fasta(const char *path, QueryFile *q) {
+    std::vector<char> buf;
+    if (!slurp(path, &buf)) {
+        set_error("cannot read query file %s", path);
+        return kErrIO;
+    }
+    std::string name;
+    unsigned long noff = 0, soff = 0;
+    size_t i = 0;
+    while (i < buf.size()) {
+        size_t e = i;
+        while (e < buf.size() && buf[e] != '\n') ++e;
+        size_t next = e < buf.size() ? e + 1 : e;
+        if (e > i) {
+            if (buf[i] == '>') {
+                name.assign(buf.data() + i + 1, e - i - 1);
+                noff = (unsigned long)(next - (e - i) - 1);
+                soff = (unsigned long)next;
+            } else {
+                q->off.push_back((int64_t)q->bases.size());
+                q->len.push_back((int32_t)(e - i));
+                q->bases.insert(q->bases.end(), buf.begin() + i, buf.begin() + e);
+                q->names.push_back(name);
+                q->name_off.push_back(noff);
+                q->seq_off.push_back(soff);
+            }
+        }
+        i = next;
+    }
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Index
+// ---------------------------------------------------------------------------
+struct IndexObj {
+    int k = 0, thr = 0, nseq = 0, max_actual = 0;
+    int64_t nhash = 0, ntup = 0, total = 0;
+    DevArr seq, seq_off, seq_len, tup0, count, start, actual, cursor, bucket, entry;
+    std::vector<int64_t> h_off, h_len;
+    std::vector<std::string> names;
+
+    IndexDev dev() const {
+        IndexDev d;
+        d.seq = seq.as<const char>();
+        d.seq_off = seq_off.as<const int64_t>();
+        d.seq_len = seq_len.as<const int64_t>();
+        d.tup0 = tup0.as<const int64_t>();
+        d.nseq = nseq;
+        d.k = k;
+        d.thr = thr;
+        d.nhash = nhash;
+        d.count = count.as<int32_t>();
+        d.start = start.as<int32_t>();
+        d.actual = actual.as<int32_t>();
+        d.cursor = cursor.as<int32_t>();
+        d.bucket = bucket.as<int2>();
+        d.entry = entry.as<uint64_t>();
+        return d;
+    }
+};
+
+static int64_t next_pow2_64(int64_t n) {
+    int64_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+static int index_build(const char *bases, const int64_t *off, const int64_t *len, int nseq, int k,
+                       int thr, IndexObj *ix) {
+    if (k < 1 || k > kMaxSeeds) {
+        set_error("seed length %d outside 1..%d", k, kMaxSeeds);
+        return kErrInvalid;
+    }
+    if (nseq < 0 || nseq > kMaxIndexSeqs) {
+        set_error("%d sequences: the index holds at most %d (refIdx is a signed char)", nseq,
+                  kMaxIndexSeqs);
+        return kErrCapacity;
+    }
+    int rc;
+    ix->k = k;
+    ix->thr = thr;
+    ix->nseq = nseq;
+    ix->nhash = (int64_t)1 << (2 * k);
+    ix->h_off.assign(nseq, 0);
+    ix->h_len.assign(nseq, 0);
+    std::vector<int64_t> tup0(std::max(nseq, 1), 0);
+    int64_t nb = 0, nt = 0;
+    for (int s = 0; s < nseq; ++s) {
+        if (len[s] < 0 || len[s] > INT32_MAX) {
+            set_error("sequence %d length %lld outside 0..2^31-1", s, (long long)len[s]);
+            return kErrCapacity;
+        }
+        ix->h_off[s] = nb;
+        ix->h_len[s] = len[s];
+        tup0[s] = nt;
+        nb += len[s];
+        nt += len[s] / k;
+    }
+    ix->ntup = nt;
+    if ((rc = ix->seq.alloc((size_t)nb + 64))) return rc;
+    for (int s = 0; s < nseq; ++s)
+        if (len[s] > 0)
+            SWIFT_HIP_TRY(hipMemcpy(ix->seq.as<char>() + ix->h_off[s], bases + off[s], (size_t)len[s],
+                                    hipMemcpyHostToDevice));
+    if ((rc = ix->seq_off.alloc(8 * (size_t)std::max(nseq, 1)))) return rc;
+    if ((rc = ix->seq_len.alloc(8 * (size_t)std::max(nseq, 1)))) return rc;
+    if ((rc = ix->tup0.alloc(8 * (size_t)std::max(nseq, 1)))) return rc;
+    if (nseq > 0) {
+        SWIFT_HIP_TRY(hipMemcpy(ix->seq_off.ptr, ix->h_off.data(), 8 * nseq, hipMemcpyHostToDevice));
+        SWIFT_HIP_TRY(hipMemcpy(ix->seq_len.ptr, ix->h_len.data(), 8 * nseq, hipMemcpyHostToDevice));
+        SWIFT_HIP_TRY(hipMemcpy(ix->tup0.ptr, tup0.data(), 8 * nseq, hipMemcpyHostToDevice));
+    }
+    const size_t nh = (size_t)ix->nhash;
+    if ((rc = ix->count.alloc(4 * nh))) return rc;
+    if ((rc = ix->start.alloc(4 * nh))) return rc;
+    if ((rc = ix->actual.alloc(4 * nh))) return rc;
+    if ((rc = ix->cursor.alloc(4 * nh))) return rc;
+    if ((rc = ix->bucket.alloc(8 * nh))) return rc;
+    SWIFT_HIP_TRY(hipMemset(ix->count.ptr, 0, 4 * nh));
+
+    const int nblocks = (int)((ix->nhash + kScanChunk - 1) / kScanChunk);
+    DevArr block_sum, small, mid_list, big_list, big_scratch;
+    if ((rc = block_sum.alloc(8 * (size_t)nblocks))) return rc;
+    if ((rc = small.alloc(64))) return rc;
+    SWIFT_HIP_TRY(hipMemset(small.ptr, 0, 64));
+    IndexBuildScratch sc{};
+    sc.block_sum = block_sum.as<int64_t>();
+    sc.total = small.as<int64_t>();
+    sc.max_actual = small.as<int32_t>() + 2;
+    sc.mid_count = small.as<int32_t>() + 3;
+    sc.big_count = small.as<int32_t>() + 4;
+    IndexDev d = ix->dev();
+    SWIFT_HIP_TRY(launch_index_build(d, ix->ntup, sc, nullptr));
+    int32_t head[6];
+    SWIFT_HIP_TRY(hipMemcpy(head, small.ptr, sizeof(head), hipMemcpyDeviceToHost));
+    int64_t total;
+    memcpy(&total, head, 8);
+    ix->total = total;
+    ix->max_actual = head[2];
+    if (total >= INT32_MAX) {
+        set_error("%lld kept tuples: bucket starts are 32-bit", (long long)total);
+        return kErrCapacity;
+    }
+    if ((rc = ix->entry.alloc(8 * (size_t)(total + 1)))) return rc;
+    SWIFT_HIP_TRY(hipMemset(ix->entry.ptr, 0xFF, 8));  // sentinel (-1, -1)
+    const int64_t nlist = total / (kSmallBucket + 1) + 1;
+    if ((rc = mid_list.alloc(4 * (size_t)nlist))) return rc;
+    if ((rc = big_list.alloc(4 * (size_t)nlist))) return rc;
+    sc.mid_list = mid_list.as<int32_t>();
+    sc.big_list = big_list.as<int32_t>();
+    if (ix->max_actual > kWaveBucket) {
+        sc.big_cap = (int32_t)next_pow2_64(ix->max_actual);
+        int64_t slots = ((int64_t)256 << 20) / ((int64_t)sc.big_cap * 8);
+        sc.big_slots = (int32_t)std::max<int64_t>(1, std::min<int64_t>(slots, 256));
+        if ((rc = big_scratch.alloc((size_t)sc.big_slots * sc.big_cap * 8))) return rc;
+        sc.big_scratch = big_scratch.as<uint64_t>();
+    }
+    d = ix->dev();
+    SWIFT_HIP_TRY(launch_index_fill(d, ix->ntup, sc, nullptr));
+    SWIFT_HIP_TRY(hipDeviceSynchronize());
+    return kOk;
+}
+
+static int index_export(const IndexObj &ix, int32_t *start, int32_t *count, int32_t *actual,
+                        int8_t *ref_idx, int32_t *ref_pos) {
+    const size_t nh = (size_t)ix.nhash;
+    if (start) SWIFT_HIP_TRY(hipMemcpy(start, ix.start.ptr, 4 * nh, hipMemcpyDeviceToHost));
+    if (count) SWIFT_HIP_TRY(hipMemcpy(count, ix.count.ptr, 4 * nh, hipMemcpyDeviceToHost));
+    if (actual) SWIFT_HIP_TRY(hipMemcpy(actual, ix.actual.ptr, 4 * nh, hipMemcpyDeviceToHost));
+    if (ref_idx || ref_pos) {
+        std::vector<uint64_t> e((size_t)ix.total + 1);
+        SWIFT_HIP_TRY(hipMemcpy(e.data(), ix.entry.ptr, 8 * e.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < e.size(); ++i) {
+            if (ref_idx) ref_idx[i] = i ? (int8_t)(e[i] >> 32) : -1;
+            if (ref_pos) ref_pos[i] = i ? (int32_t)(uint32_t)e[i] : -1;
+        }
+    }
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Mapping
+// ---------------------------------------------------------------------------
+struct MapWork {
+    DevArr ref, shift, pos, nbest, nbig, nhits, strand_seq, ovl_list, flags, scratch;
+    DevArr pair0, qoff, roff, coff, qlen, rlen;
+};
+
+static std::mutex g_p1_mu;
+static MapWork &map_work() {
+    static MapWork *w = new MapWork;  // lives until the process ends
+    return *w;
+}
+
+static int ensure(DevArr &a, size_t bytes) { return bytes <= a.bytes ? (int)kOk : a.alloc(bytes); }
+
+// Fills `m` for a launch over `rd`; out arrays may come from the caller
+// (any of them null: the workspace's are used).
+static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len, int max_hits,
+                       MapArgs *m, MapWork &w, bool want_strands) {
+    if (max_read_len < 0 || max_read_len > kMaxMapRead) {
+        set_error("reads up to %d bases (got max_read_len %d)", kMaxMapRead, max_read_len);
+        return kErrCapacity;
+    }
+    if (max_hits < 0) {
+        set_error("negative max_hits");
+        return kErrInvalid;
+    }
+    const int ns = 2 * rd.nreads;
+    const int stride = std::max(max_hits, 1);
+    int rc;
+    const int nq = std::max(max_read_len - ix.k + 1, 0);
+    const int64_t bound = (int64_t)nq * ix.max_actual;
+    if (bound >= INT32_MAX) {
+        set_error("up to %lld hits per strand: beyond the 32-bit hit counters", (long long)bound);
+        return kErrCapacity;
+    }
+    if ((rc = ensure(w.ref, (size_t)ns * stride + 64))) return rc;
+    if ((rc = ensure(w.shift, 4 * ((size_t)ns * stride + 16)))) return rc;
+    if ((rc = ensure(w.pos, 4 * ((size_t)ns * stride + 16)))) return rc;
+    if ((rc = ensure(w.nbest, 4 * ((size_t)ns + 16)))) return rc;
+    if ((rc = ensure(w.nbig, 4 * ((size_t)ns + 16)))) return rc;
+    if ((rc = ensure(w.nhits, 4 * ((size_t)ns + 16)))) return rc;
+    if ((rc = ensure(w.ovl_list, 4 * ((size_t)ns + 16)))) return rc;
+    if ((rc = ensure(w.flags, 64))) return rc;
+    const int qstride = (max_read_len + 15) & ~15;
+    if (want_strands && (rc = ensure(w.strand_seq, (size_t)ns * qstride + 64))) return rc;
+    int slots = 0;
+    int64_t cap = 0;
+    if (bound > kMapCap) {
+        // the workgroup sort pads a strand's keys to a power of two in its slot
+        cap = next_pow2_64(bound);
+        slots = (int)std::max<int64_t>(1, std::min<int64_t>(64, (((int64_t)256 << 20) / (cap * 8))));
+        if ((rc = ensure(w.scratch, (size_t)slots * cap * 8))) return rc;
+    }
+    m->ix = ix.dev();
+    m->rd = rd;
+    m->out = MapOut{w.ref.as<int8_t>(), w.shift.as<int32_t>(), w.pos.as<int32_t>(),
+                    w.nbest.as<int32_t>(), w.nbig.as<int32_t>(), w.nhits.as<int32_t>(), stride};
+    m->max_hits = max_hits;
+    m->max_len = max_read_len;
+    m->strand_seq = want_strands ? w.strand_seq.as<char>() : nullptr;
+    m->qstride = qstride;
+    m->scratch = w.scratch.as<uint64_t>();
+    m->scratch_cap = cap;
+    m->scratch_slots = slots;
+    m->ovl_list = w.ovl_list.as<int32_t>();
+    m->ovl_count = w.flags.as<int32_t>();
+    m->error = w.flags.as<int32_t>() + 1;
+    return kOk;
+}
+
+static int run_map(const MapArgs &m, hipStream_t st) {
+    SWIFT_HIP_TRY(launch_fill(m.ovl_count, 2, 0, st));  // overflow count, error flags
+    SWIFT_HIP_TRY(launch_map(m, st));
+    return kOk;
+}
+
+static int map_error(int flags) {
+    if (flags & 2) {
+        set_error("a read is longer than max_read_len");
+        return kErrCapacity;
+    }
+    if (flags & 1) {
+        set_error("a strand's hits exceeded the overflow scratch");
+        return kErrCapacity;
+    }
+    return kOk;
+}
+
+static int map_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len, int max_hits,
+                     int8_t *ref, int32_t *shift, int32_t *pos, int32_t *nbest, int32_t *nbig,
+                     int32_t *nhits, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_p1_mu);
+    MapWork &w = map_work();
+    MapArgs m;
+    int rc = prepare_map(ix, rd, max_read_len, max_hits, &m, w, false);
+    if (rc) return rc;
+    m.out.ref = ref;
+    m.out.shift = shift;
+    m.out.pos = pos;
+    m.out.nbest = nbest;
+    if (nbig) m.out.nbig = nbig;
+    if (nhits) m.out.nhits = nhits;
+    if ((rc = run_map(m, st))) return rc;
+    int32_t flags[2];
+    SWIFT_HIP_TRY(hipMemcpyAsync(flags, m.ovl_count, 8, hipMemcpyDeviceToHost, st));
+    SWIFT_HIP_TRY(hipStreamSynchronize(st));
+    return map_error(flags[1]);
+}
+
+// Phase 1 -> phase 2.
+static int align_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len, int max_hits,
+                       const Scoring &sc, const swift_records &rec, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_p1_mu);
+    if (max_read_len > kMaxBatchQ) {
+        set_error("the pipeline aligns reads up to %d bases (got %d)", kMaxBatchQ, max_read_len);
+        return kErrCapacity;
+    }
+    constexpr int kBefore = 100, kWin = 300;
+    const int64_t cap = 2 * (int64_t)rd.nreads * std::max(max_hits, 0);
+    if (cap > rec.capacity || cap >= INT32_MAX) {
+        set_error("record capacity %d below 2 * nreads * max_hits = %lld", rec.capacity,
+                  (long long)cap);
+        return kErrCapacity;
+    }
+    if (rec.cons_stride < max_read_len + kWin + 2) {
+        set_error("cons_stride %lld below max_read_len + 302", (long long)rec.cons_stride);
+        return kErrInvalid;
+    }
+    MapWork &w = map_work();
+    MapArgs m;
+    int rc = prepare_map(ix, rd, max_read_len, max_hits, &m, w, true);
+    if (rc) return rc;
+    const size_t c = (size_t)std::max<int64_t>(cap, 1);
+    if ((rc = ensure(w.pair0, 4 * (2 * (size_t)rd.nreads + 16)))) return rc;
+    if ((rc = ensure(w.qoff, 8 * c))) return rc;
+    if ((rc = ensure(w.roff, 8 * c))) return rc;
+    if ((rc = ensure(w.coff, 8 * c))) return rc;
+    if ((rc = ensure(w.qlen, 4 * c))) return rc;
+    if ((rc = ensure(w.rlen, 4 * c))) return rc;
+    if ((rc = run_map(m, st))) return rc;
+    PairsDev p;
+    p.npairs = rec.nrecords;
+    p.pair0 = w.pair0.as<int32_t>();
+    p.read = rec.read;
+    p.strand = rec.strand;
+    p.ref = rec.ref_idx;
+    p.shift = rec.shift;
+    p.pos = rec.ref_pos;
+    p.nbig = rec.nbiggest;
+    p.rdist = rec.ref_distance;
+    p.qoff = w.qoff.as<int64_t>();
+    p.roff = w.roff.as<int64_t>();
+    p.coff = w.coff.as<int64_t>();
+    p.qlen = w.qlen.as<int32_t>();
+    p.rlen = w.rlen.as<int32_t>();
+    p.capacity = (int)cap;
+    p.cstride = rec.cons_stride;
+    SWIFT_HIP_TRY(launch_records(m, p, kBefore, kWin, st));
+    BatchIO io;
+    io.qseq = m.strand_seq;
+    io.qoff = p.qoff;
+    io.qlen = p.qlen;
+    io.rseq = m.ix.seq;
+    io.roff = p.roff;
+    io.rlen = p.rlen;
+    io.npairs = (int)cap;
+    io.maxq = max_read_len;
+    io.maxr = kWin;
+    io.score = rec.score;
+    io.start = rec.start;
+    io.end = rec.end;
+    io.alen = rec.alen;
+    io.qcons = rec.qry_cons;
+    io.rcons = rec.ref_cons;
+    io.cons_off = p.coff;
+    if ((rc = sw_batch_run(io, sc, st, true))) return rc;
+    SWIFT_HIP_TRY(launch_choose_best(p, rd.nreads, rec.score, rec.keep, rec.ntied, st));
+    int32_t flags[2];
+    SWIFT_HIP_TRY(hipMemcpyAsync(flags, m.ovl_count, 8, hipMemcpyDeviceToHost, st));
+    SWIFT_HIP_TRY(hipStreamSynchronize(st));
+    return map_error(flags[1]);
+}
+
+// ---------------------------------------------------------------------------
+// The reference's module state (src/lookupTable5.cu:39-51): one index
+// shared by lookupTable5MapQry calls.
+// ---------------------------------------------------------------------------
+static IndexObj *g_lt5 = nullptr;
+static int g_lt5_max_hits = 0;
+
+static int build_from_fasta(const char *path, int k, int thr, IndexObj **out) {
+    RefSeqs r;
+    int rc = read_reference_fasta(path, &r);
+    if (rc) return rc;
+    std::unique_ptr<IndexObj> ix(new IndexObj);
+    rc = index_build(r.bases.data(), r.off.data(), r.len.data(), (int)r.off.size(), k, thr,
+                     ix.get());
+    if (rc) return rc;
+    ix->names = std::move(r.names);
+    *out = ix.release();
+    return kOk;
+}
+
+static void die(const char *fn) {
+    fprintf(stderr, "Error: %s: %s\n", fn, last_error());
+    exit(1);
+}
+
+}  // namespace swift
+
+using namespace swift;
+
+struct swift_index {
+    IndexObj ix;
+};
+
+extern "C" {
+
+void preprocessCreate(void) {}
+void preprocessDelete(void) {}
+
+int getHash(char *s, int length) {
+    int h = 0;
+    for (int i = length - 1; i >= 0; --i) {
+        int c;
+        switch (s[i]) {
+        case 'C': case 'c': c = 1; break;
+        case 'G': case 'g': c = 2; break;
+        case 'T': case 't': c = 3; break;
+        default: c = 0;
+        }
+        h = h * 4 + c;
+    }
+    return h;
+}
+
+void qryGetReverseComplement(char *seq, int seqLength, char *revComp) {
+    for (int i = 0; i < seqLength; ++i) {
+        char c = seq[seqLength - 1 - i];
+        revComp[i] = c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
+    }
+    revComp[seqLength < 0 ? 0 : seqLength] = '\0';
+}
+
+void lookupTable5Create(const char *refFile, int seedLen, int maxHitsPerQry,
+                        int tupleIgnoreThreshold, int *totalTuples) {
+    std::lock_guard<std::mutex> lk(g_p1_mu);
+    delete g_lt5;
+    g_lt5 = nullptr;
+    IndexObj *ix = nullptr;
+    if (build_from_fasta(refFile, seedLen, tupleIgnoreThreshold, &ix)) die(__func__);
+    g_lt5 = ix;
+    g_lt5_max_hits = maxHitsPerQry;
+    if (totalTuples) *totalTuples = (int)ix->total;
+}
+
+int lookupTable5MapQry(const Query *query, unsigned int qryLen, unsigned int isRevComp,
+                       char *refIdx_bestHits, int *shift_bestHits, int *refPos_bestHits) {
+    if (!g_lt5) {
+        set_error("lookupTable5MapQry before lookupTable5Create");
+        die(__func__);
+    }
+    const char *seq = isRevComp == 1 ? query->revCompSeq : query->seq;
+    const int L = (int)qryLen;
+    const int mh = g_lt5_max_hits;
+    DevArr dseq, doff, dlen, dref, dsh, dpos, dn;
+    int64_t off0 = 0;
+    int32_t len0 = L;
+    if (dseq.alloc(std::max(L, 1)) || doff.alloc(8) || dlen.alloc(4) ||
+        dref.alloc(2 * std::max(mh, 1)) || dsh.alloc(8 * std::max(mh, 1)) ||
+        dpos.alloc(8 * std::max(mh, 1)) || dn.alloc(8))
+        die(__func__);
+    if (L > 0 && hipMemcpy(dseq.ptr, seq, L, hipMemcpyHostToDevice) != hipSuccess) die(__func__);
+    if (hipMemcpy(doff.ptr, &off0, 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dlen.ptr, &len0, 4, hipMemcpyHostToDevice) != hipSuccess)
+        die(__func__);
+    ReadsDev rd{dseq.as<char>(), doff.as<int64_t>(), dlen.as<int32_t>(), 1};
+    // strand 0 of this "read" is the requested sequence itself
+    if (map_reads(*g_lt5, rd, std::max(L, 0), mh, dref.as<int8_t>(), dsh.as<int32_t>(),
+                  dpos.as<int32_t>(), dn.as<int32_t>(), nullptr, nullptr, nullptr))
+        die(__func__);
+    int32_t n[2];
+    if (hipMemcpy(n, dn.ptr, 8, hipMemcpyDeviceToHost) != hipSuccess) die(__func__);
+    if (n[0] > 0) {
+        if (hipMemcpy(refIdx_bestHits, dref.ptr, n[0], hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(shift_bestHits, dsh.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(refPos_bestHits, dpos.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess)
+            die(__func__);
+    }
+    return n[0];
+}
+
+void lookupTable5Delete(void) {
+    std::lock_guard<std::mutex> lk(g_p1_mu);
+    delete g_lt5;
+    g_lt5 = nullptr;
+}
+
+int mapHits6GetBestHits(int numBestHits, char *refIdx, int *shift, int *refPos, int size,
+                        char *refIdx_bestMatch, int *shift_bestMatch, int *refPos_bestMatch) {
+    if (size <= 0) return 0;  // src/mapHits6.cu:45-46
+    const int mh = std::max(numBestHits, 0);
+    DevArr dref, dsh, dpos, oref, osh, opos, on, kbuf;
+    if (dref.alloc(size) || dsh.alloc(4 * (size_t)size) || dpos.alloc(4 * (size_t)size) ||
+        oref.alloc(std::max(mh, 1)) || osh.alloc(4 * std::max(mh, 1)) ||
+        opos.alloc(4 * std::max(mh, 1)) || on.alloc(8) ||
+        kbuf.alloc(16 * (size_t)next_pow2_64(size)))
+        die(__func__);
+    if (hipMemcpy(dref.ptr, refIdx, size, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dsh.ptr, shift, 4 * (size_t)size, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dpos.ptr, refPos, 4 * (size_t)size, hipMemcpyHostToDevice) != hipSuccess)
+        die(__func__);
+    if (launch_best_hits(dref.as<int8_t>(), dsh.as<int32_t>(), dpos.as<int32_t>(), size, mh,
+                         oref.as<int8_t>(), osh.as<int32_t>(), opos.as<int32_t>(),
+                         on.as<int32_t>(), on.as<int32_t>() + 1, kbuf.ptr, nullptr) != hipSuccess)
+        die(__func__);
+    int32_t n[2];
+    if (hipMemcpy(n, on.ptr, 8, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(refIdx, dref.ptr, size, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(shift, dsh.ptr, 4 * (size_t)size, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(refPos, dpos.ptr, 4 * (size_t)size, hipMemcpyDeviceToHost) != hipSuccess)
+        die(__func__);
+    if (n[0] > 0) {
+        if (hipMemcpy(refIdx_bestMatch, oref.ptr, n[0], hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(shift_bestMatch, osh.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(refPos_bestMatch, opos.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess)
+            die(__func__);
+    }
+    return n[0];
+}
+
+void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName, int seedLen,
+                    char *matchFile, unsigned int maxNumHits, int *numMatches, int numThreads,
+                    int tupleIgnoreThres) {
+    (void)qryLen;
+    (void)numThreads;
+    // src/search.cu:279-281
+    if (qryFileName == NULL || refFileName == NULL || seedLen <= 0 || matchFile == NULL ||
+        numQrs <= 0)
+        return;
+    IndexObj *ix = nullptr;
+    if (build_from_fasta(refFileName, seedLen, tupleIgnoreThres, &ix)) die(__func__);
+    std::unique_ptr<IndexObj> hold(ix);
+    QueryFile q;
+    if (read_query_fasta(qryFileName, &q)) die(__func__);
+    const int nreads = std::min<int>(numQrs, (int)q.len.size());
+    int maxlen = 0;
+    for (int r = 0; r < nreads; ++r) maxlen = std::max(maxlen, q.len[r]);
+    if (maxlen > kMaxMapRead) {
+        set_error("reads up to %d bases (found %d)", kMaxMapRead, maxlen);
+        die(__func__);
+    }
+    FILE *out = fopen(matchFile, "w");
+    if (!out) {
+        set_error("cannot write %s", matchFile);
+        die(__func__);
+    }
+    const int mh = (int)maxNumHits;
+    const int stride = std::max(mh, 1);
+    const int batch = 1 << 18;
+    int total_lines = 0;
+    DevArr dseq, doff, dlen, dref, dsh, dpos, dn;
+    std::vector<int8_t> href;
+    std::vector<int32_t> hsh, hpos, hn;
+    for (int r0 = 0; r0 < nreads; r0 += batch) {
+        const int nr = std::min(batch, nreads - r0);
+        const int64_t b0 = q.off[r0];
+        const int64_t b1 = q.off[r0 + nr - 1] + q.len[r0 + nr - 1];
+        std::vector<int64_t> off(nr);
+        for (int i = 0; i < nr; ++i) off[i] = q.off[r0 + i] - b0;
+        const size_t ns = 2 * (size_t)nr;
+        if (dseq.alloc(std::max<int64_t>(b1 - b0, 1)) || doff.alloc(8 * nr) ||
+            dlen.alloc(4 * nr) || dref.alloc(ns * stride) || dsh.alloc(4 * ns * stride) ||
+            dpos.alloc(4 * ns * stride) || dn.alloc(4 * ns))
+            die(__func__);
+        if ((b1 > b0 &&
+             hipMemcpy(dseq.ptr, q.bases.data() + b0, b1 - b0, hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(doff.ptr, off.data(), 8 * nr, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dlen.ptr, q.len.data() + r0, 4 * nr, hipMemcpyHostToDevice) != hipSuccess)
+            die(__func__);
+        ReadsDev rd{dseq.as<char>(), doff.as<int64_t>(), dlen.as<int32_t>(), nr};
+        if (map_reads(*ix, rd, maxlen, mh, dref.as<int8_t>(), dsh.as<int32_t>(), dpos.as<int32_t>(),
+                      dn.as<int32_t>(), nullptr, nullptr, nullptr))
+            die(__func__);
+        href.resize(ns * stride);
+        hsh.resize(ns * stride);
+        hpos.resize(ns * stride);
+        hn.resize(ns);
+        if (hipMemcpy(href.data(), dref.ptr, href.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hsh.data(), dsh.ptr, 4 * hsh.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hpos.data(), dpos.ptr, 4 * hpos.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hn.data(), dn.ptr, 4 * hn.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            die(__func__);
+        // printHeuristicMatches4 (src/search.cu:2230-2240)
+        for (int i = 0; i < nr; ++i) {
+            const int r = r0 + i;
+            for (int s = 0; s < 2; ++s) {
+                const size_t sidx = 2 * (size_t)i + s;
+                for (int h = 0; h < hn[sidx]; ++h) {
+                    const size_t o = sidx * stride + h;
+                    fprintf(out, "%s\t%d\t%d\t%d\t%d\t%lu\t%lu\n", q.names[r].c_str(), s,
+                            (int)href[o], hsh[o], hpos[o], q.name_off[r], q.seq_off[r]);
+                    ++total_lines;
+                }
+            }
+        }
+    }
+    fclose(out);
+    if (numMatches) *numMatches = total_lines;
+}
+
+// ---- status-returning API --------------------------------------------------
+int swift_index_build(const char *seq, const int64_t *off, const int64_t *len, int nseq,
+                      int seed_len, int tuple_ignore_threshold, swift_index **out) {
+    if (!out || (nseq > 0 && (!seq || !off || !len))) {
+        set_error("null argument");
+        return kErrInvalid;
+    }
+    std::lock_guard<std::mutex> lk(g_p1_mu);
+    std::unique_ptr<swift_index> h(new swift_index);
+    int rc = index_build(seq, off, len, nseq, seed_len, tuple_ignore_threshold, &h->ix);
+    if (rc) return rc;
+    h->ix.names.assign(nseq, std::string());
+    *out = h.release();
+    return kOk;
+}
+
+int swift_index_build_fasta(const char *path, int seed_len, int tuple_ignore_threshold,
+                            swift_index **out) {
+    if (!out || !path) {
+        set_error("null argument");
+        return kErrInvalid;
+    }
+    std::lock_guard<std::mutex> lk(g_p1_mu);
+    RefSeqs r;
+    int rc = read_reference_fasta(path, &r);
+    if (rc) return rc;
+    std::unique_ptr<swift_index> h(new swift_index);
+    rc = index_build(r.bases.data(), r.off.data(), r.len.data(), (int)r.off.size(), seed_len,
+                     tuple_ignore_threshold, &h->ix);
+    if (rc) return rc;
+    h->ix.names = std::move(r.names);
+    *out = h.release();
+    return kOk;
+}
+
+int swift_index_free(swift_index *ix) {
+    std::lock_guard<std::mutex> lk(g_p1_mu);
+    delete ix;
+    return kOk;
+}
+
+int swift_index_info(const swift_index *ix, int *nseq, int64_t *total_tuples, int64_t *nhash,
+                     int *max_bucket) {
+    if (!ix) {
+        set_error("no index");
+        return kErrNoIndex;
+    }
+    if (nseq) *nseq = ix->ix.nseq;
+    if (total_tuples) *total_tuples = ix->ix.total;
+    if (nhash) *nhash = ix->ix.nhash;
+    if (max_bucket) *max_bucket = ix->ix.max_actual;
+    return kOk;
+}
+
+int swift_index_export(const swift_index *ix, int32_t *start, int32_t *count, int32_t *actual,
+                       int8_t *ref_idx, int32_t *ref_pos) {
+    if (!ix) {
+        set_error("no index");
+        return kErrNoIndex;
+    }
+    return index_export(ix->ix, start, count, actual, ref_idx, ref_pos);
+}
+
+int swift_index_sequence(const swift_index *ix, int s, int64_t *len, const char **name) {
+    if (!ix) {
+        set_error("no index");
+        return kErrNoIndex;
+    }
+    if (s < 0 || s >= ix->ix.nseq) {
+        set_error("sequence %d outside 0..%d", s, ix->ix.nseq - 1);
+        return kErrInvalid;
+    }
+    if (len) *len = ix->ix.h_len[s];
+    if (name) *name = ix->ix.names[s].c_str();
+    return kOk;
+}
+
+int swift_map_reads(const swift_index *ix, const char *reads_d, const int64_t *off_d,
+                    const int32_t *len_d, int nreads, int max_read_len, int max_hits,
+                    int8_t *ref_idx_d, int32_t *shift_d, int32_t *ref_pos_d, int32_t *nbest_d,
+                    int32_t *nbiggest_d, int32_t *nhits_d, void *stream) {
+    if (!ix) {
+        set_error("no index");
+        return kErrNoIndex;
+    }
+    if (nreads < 0 || (nreads > 0 && (!reads_d || !off_d || !len_d || !ref_idx_d || !shift_d ||
+                                      !ref_pos_d || !nbest_d))) {
+        set_error("null or negative argument");
+        return kErrInvalid;
+    }
+    if (nreads == 0) return kOk;
+    ReadsDev rd{reads_d, off_d, len_d, nreads};
+    return map_reads(ix->ix, rd, max_read_len, max_hits, ref_idx_d, shift_d, ref_pos_d, nbest_d,
+                     nbiggest_d, nhits_d, (hipStream_t)stream);
+}
+
+int swift_align_reads(const swift_index *ix, const char *reads_d, const int64_t *off_d,
+                      const int32_t *len_d, int nreads, int max_read_len, int max_hits,
+                      int match, int mismatch, int gap_open, int gap_ext,
+                      const swift_records *rec, void *stream) {
+    if (!ix) {
+        set_error("no index");
+        return kErrNoIndex;
+    }
+    if (!rec || nreads < 0 || (nreads > 0 && (!reads_d || !off_d || !len_d))) {
+        set_error("null or negative argument");
+        return kErrInvalid;
+    }
+    Scoring sc{match, mismatch, gap_open, gap_ext};
+    ReadsDev rd{reads_d, off_d, len_d, nreads};
+    if (nreads == 0) {
+        int32_t zero = 0;
+        SWIFT_HIP_TRY(hipMemcpy(rec->nrecords, &zero, 4, hipMemcpyHostToDevice));
+        return kOk;
+    }
+    return align_reads(ix->ix, rd, max_read_len, max_hits, sc, *rec, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -267,7 +267,8 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
 
     auto pass_args = [&](int pass, const PairResult *prev, int f32, BinArgs &b, TaskArgs &t) {
         b = BinArgs{io.rseq, io.roff, io.qlen, io.rlen, prev, n, f32, w.bin_of.as<int32_t>(),
-                    bins + (2 * pass) * kNumBins, cnt + kCntError};
+                    bins + (2 * pass) * kNumBins, cnt + kCntError, io.score, io.start, io.end,
+                    io.alen};
         t = TaskArgs{bins + (2 * pass) * kNumBins, bins + (2 * pass + 1) * kNumBins,
                      segs + (3 * pass) * NS, segs + (3 * pass + 1) * NS,
                      segs + (3 * pass + 2) * NS, cnt + 4 * pass, cnt + 4 * pass + 1};

@@ -664,6 +664,18 @@ __global__ void sw_bin_classify(BinArgs a) {
         atomicOr(a.error, 1);
         return;
     }
+    if (ql == 0 || rl == 0) {
+        // all-zero matrix: start 0, end -1, length 0 (src/smithWaterman.cu
+        // :50,333-334)
+        if (!a.prev) {
+            a.score_out[p] = 0;
+            a.start_out[p] = 0;
+            a.end_out[p] = -1;
+            a.alen_out[p] = 0;
+        }
+        a.bin_of[p] = -1;
+        return;
+    }
     int cls;
     if (a.prev) {
         if (!(a.prev[p].flags & 1)) { a.bin_of[p] = -1; return; }

@@ -101,6 +101,9 @@ struct BinArgs {
     int32_t *bin_of;          // per pair bin (or -1 when not in this pass)
     int32_t *bin_count;       // kNumBins
     int32_t *error;           // set to nonzero when a pair's lengths are out of range
+    // empty pairs (query or reference of length 0) get their all-zero result
+    // here and take no part in the kernels
+    int32_t *score_out, *start_out, *end_out, *alen_out;
 };
 
 struct TaskArgs {

@@ -1,0 +1,243 @@
+"""Phase-1 parity on the GPU: the device index, the device mapper, the
+reference-named drop-ins and the phase-1 -> phase-2 pipeline against the
+oracle (tie rule "first") and the reference test suite's known answers."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from test_oracle_golden import _check_choice, parse_fasta
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_vectors.json")))
+
+
+@pytest.fixture(scope="module")
+def S():
+    from swift_amd import search
+    return search
+
+
+def assert_index_equal(got, want):
+    for key in ("start", "count", "actual", "refIdx", "refPos"):
+        assert np.array_equal(got[key], want[key]), key
+
+
+def rand_genome(rnd, nseq, lo, hi, alphabet=b"ACGT", repeats=0):
+    seqs = []
+    for _ in range(nseq):
+        s = bytearray(rnd.choice(alphabet) for _ in range(rnd.randint(lo, hi)))
+        for _ in range(repeats):  # tandem arrays make large buckets and ties
+            unit = bytes(rnd.choice(b"ACGT") for _ in range(rnd.randint(2, 30)))
+            span = min(len(s), len(unit) * rnd.randint(3, 60))
+            p = rnd.randint(0, len(s) - span)
+            s[p:p + span] = (unit * (span // len(unit) + 1))[:span]
+        seqs.append(bytes(s))
+    return seqs
+
+
+# ------------------------------------------------------------------ index
+@pytest.mark.parametrize("k,thr", [(4, 3), (8, 4), (12, 1000), (6, 40)])
+def test_index_vs_oracle(S, orc, k, thr):
+    rnd = random.Random(k * 100 + thr)
+    seqs = rand_genome(rnd, 5, 0, 30000, b"ACGTACGTACGTNnacgt", repeats=6)
+    want = orc.Index(seqs, k, thr)
+    got = S.Index.from_sequences(seqs, k, thr)
+    assert got.info()["total"] == want.total
+    assert_index_equal(got.arrays(), want.arrays())
+
+
+def test_index_large_buckets(S, orc):
+    """Buckets past the register and wave sorts (thr 5000, tandem repeats)."""
+    rnd = random.Random(5)
+    unit = b"ACGTTGCAAC"
+    seqs = [unit * 3000 + bytes(rnd.choice(b"ACGT") for _ in range(5000)), unit * 1500]
+    want = orc.Index(seqs, 10, 5000)
+    got = S.Index.from_sequences(seqs, 10, 5000)
+    assert got.info()["max_bucket"] > 1024
+    assert_index_equal(got.arrays(), want.arrays())
+
+
+@pytest.mark.parametrize("case", GOLD["lookup_table5_create"], ids=lambda c: c["source"][:44])
+def test_index_fasta_known(S, orc, case, tmp_path):
+    """lookupTable5Create known answers through the FASTA reader."""
+    p = tmp_path / "ref.fa"
+    p.write_text(case["fasta"])
+    a = S.Index.from_fasta(p, case["k"], case["thr"]).arrays()
+    for h, v in case["lookup"].items():
+        assert a["start"][int(h)] == v
+    assert list(a["refIdx"][:len(case["refIdx"])]) == case["refIdx"]
+    assert list(a["refPos"][:len(case["refPos"])]) == case["refPos"]
+    for h, v in case["count"].items():
+        assert a["count"][int(h)] == v
+    assert_index_equal(a, orc.Index(parse_fasta(case["fasta"]), case["k"], case["thr"]).arrays())
+
+
+def test_index_fasta_line_handling(S, orc, tmp_path):
+    """Any line width, blank lines and a missing final newline give the index
+    of the concatenated lines."""
+    from swift_amd import synth
+    rnd = random.Random(2)
+    seqs = rand_genome(rnd, 4, 100, 3000)
+    want = orc.Index(seqs, 8, 20).arrays()
+    for width in (7, 50, 61, 100000):
+        text = synth.to_fasta(seqs, width=width)
+        text = text.replace(b"\n", b"\n\n", 3).rstrip(b"\n")
+        p = tmp_path / f"w{width}.fa"
+        p.write_bytes(text)
+        ix = S.Index.from_fasta(p, 8, 20)
+        assert_index_equal(ix.arrays(), want)
+        assert [ix.sequence(s)[0] for s in range(len(seqs))] == [len(s) for s in seqs]
+
+
+def test_lookup_table5_create_dropin(S, orc, tmp_path):
+    case = GOLD["lookup_table5_create"][1]
+    p = tmp_path / "ref.fa"
+    p.write_text(case["fasta"])
+    tot = S.lookup_table5_create(p, case["k"], 3, case["thr"])
+    assert tot == orc.Index(parse_fasta(case["fasta"]), case["k"], case["thr"]).total
+    S.lookup_table5_delete()
+
+
+# ------------------------------------------------------------------ mapping
+def check_strands(ix_orc, reads, got, max_hits, orc):
+    for r, q in enumerate(reads):
+        for strand in (0, 1):
+            s = 2 * r + strand
+            seq = q if strand == 0 else orc.revcomp(q)
+            want, nbig, nh = ix_orc.map_query(seq, max_hits, orc.TIE_FIRST)
+            assert got.hits(s) == want, (r, strand)
+            assert got.nbiggest[s] == nbig and got.nhits[s] == nh, (r, strand)
+
+
+@pytest.mark.parametrize("max_hits", [1, 3, 10])
+def test_map_reads_vs_oracle(S, orc, max_hits):
+    from swift_amd import synth
+    rnd = random.Random(max_hits)
+    seqs = rand_genome(rnd, 6, 2000, 40000, repeats=10)
+    g = np.random.default_rng(max_hits)
+    reads, _ = synth.sample_reads(g, [np.frombuffer(s, np.uint8) for s in seqs], 400, 100,
+                                  0.01, 0.001)
+    reads += [bytes(rnd.choice(b"ACGTN") for _ in range(rnd.randint(0, 150))) for _ in range(100)]
+    ix = S.Index.from_sequences(seqs, 12, 1000)
+    got = ix.map_reads(reads, max_hits)
+    check_strands(orc.Index(seqs, 12, 1000), reads, got, max_hits, orc)
+
+
+def test_map_reads_overflow_strands(S, orc):
+    """Strands with more hits than the LDS sort holds take the scratch path."""
+    rnd = random.Random(4)
+    unit = b"ACGTTGCAAC"
+    seqs = [unit * 2000, bytes(rnd.choice(b"ACGT") for _ in range(20000)) + unit * 300]
+    reads = [unit * 20, (unit * 30)[3:250], seqs[1][100:300], unit[:5] * 40]
+    ix = S.Index.from_sequences(seqs, 10, 100000)
+    got = ix.map_reads(reads, 3)
+    assert got.nhits.max() > 1024
+    check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
+
+
+def test_map_reads_rejects_long_reads(S):
+    ix = S.Index.from_sequences([b"ACGT" * 100], 8, 10)
+    with pytest.raises(S.SwiftError):
+        ix.map_reads([b"ACGT" * 20], 3, max_read_len=40)
+
+
+@pytest.mark.parametrize("case", GOLD["lookup_table5_map"], ids=lambda c: c["source"][:44])
+def test_lookup_table5_map_qry_known(S, case, tmp_path):
+    p = tmp_path / "ref.fa"
+    p.write_text(case["fasta"])
+    S.lookup_table5_create(p, case["k"], case["max_hits"], case["thr"])
+    q = case["qry"].encode()
+    _check_choice(S.lookup_table5_map_qry(q, False), case["fwd"])
+    _check_choice(S.lookup_table5_map_qry(q, True), case["rev"])
+    S.lookup_table5_delete()
+
+
+@pytest.mark.parametrize("case", GOLD["map_hits6"], ids=lambda c: c["source"][:40])
+def test_map_hits6_known(S, case):
+    best, srt = S.map_hits6_best_hits([tuple(h) for h in case["hits"]], case["max_hits"])
+    if "sorted" in case:
+        assert [list(s) for s in srt] == case["sorted"]
+    if "best" in case:
+        assert [list(b) for b in best] == case["best"]
+    if "choose" in case:
+        _check_choice(best, case)
+
+
+def test_map_hits6_vs_oracle(S, orc):
+    rnd = random.Random(12)
+    for n in list(range(1, 40)) + [300, 3000]:
+        hits = [(rnd.randint(-3, 3), rnd.randint(-8, 8), rnd.randint(-50, 50)) for _ in range(n)]
+        mh = rnd.randint(1, 5)
+        best, srt = S.map_hits6_best_hits(hits, mh)
+        obest, _, osrt = orc.best_hits(hits, mh, orc.TIE_FIRST)
+        assert srt == osrt and best == obest, n
+
+
+@pytest.mark.parametrize("case", GOLD["search_queries3"], ids=lambda c: c["source"][:40])
+def test_search_queries3_known(S, case, tmp_path):
+    ref, qry, out = tmp_path / "r.fa", tmp_path / "q.fa", tmp_path / "m.txt"
+    ref.write_text(case["fasta"])
+    qry.write_text("".join(f">{n}\n{s}\n" for n, s in case["reads"]))
+    nm = S.search_queries3(qry, ref, out, case["k"], case["max_hits"], 1000)
+    lines = out.read_text().splitlines()
+    assert nm == len(lines)
+    if "lines" in case:
+        assert lines == case["lines"]
+    else:
+        assert set(lines) <= set(case["fwd_set"]) | set(case["rev_set"])
+        assert nm == case["num_matches"]
+
+
+# ------------------------------------------------------------------ pipeline
+REC_KEYS = ("read", "strand", "ref_idx", "shift", "ref_pos", "nbiggest", "ref_distance", "score",
+            "start", "end", "alen", "keep", "ntied", "qry_cons", "ref_cons")
+ORC_KEYS = ("read", "strand", "refIdx", "shift", "refPos", "nBiggest", "refDistance", "score",
+            "start", "end", "len", "keep", "nTied", "qry_cons", "ref_cons")
+
+
+def test_align_reads_vs_oracle_pipeline(S, orc):
+    """Reads -> both strands mapped -> 300-bp windows (N-trimmed) -> SW ->
+    best-hit choice, record by record against the oracle's CPU pipeline."""
+    from swift_amd import synth
+    rnd = random.Random(21)
+    seqs = rand_genome(rnd, 4, 5000, 60000, repeats=8)
+    seqs[1] = b"N" * 150 + seqs[1][150:-200] + b"n" * 200   # N runs at the ends
+    g = np.random.default_rng(21)
+    reads, _ = synth.sample_reads(g, [np.frombuffer(s, np.uint8) for s in seqs], 300, 100,
+                                  0.02, 0.002)
+    reads += [bytes(rnd.choice(b"ACGT") for _ in range(100)) for _ in range(20)]
+    reads += [seqs[1][:100], seqs[1][-100:], seqs[0][:60] + b"acgtn" + seqs[0][65:100]]
+    ix = S.Index.from_sequences(seqs, 12, 1000)
+    got = ix.align_reads(reads, 3)
+    want = orc.pipeline(orc.Index(seqs, 12, 1000), reads, 3)
+    assert got.n == len(want)
+    for i, w in enumerate(want):
+        g_rec = got.record(i)
+        for gk, wk in zip(REC_KEYS, ORC_KEYS):
+            assert g_rec[gk] == w[wk], (i, gk, g_rec, w)
+
+
+def test_align_reads_uneven_lengths(S, orc):
+    from swift_amd import synth
+    rnd = random.Random(33)
+    seqs = rand_genome(rnd, 3, 20000, 50000)
+    g = np.random.default_rng(33)
+    arr = [np.frombuffer(s, np.uint8) for s in seqs]
+    reads = []
+    for L in (50, 77, 150, 220, 300):
+        rr, _ = synth.sample_reads(g, arr, 40, L, 0.015, 0.0015)
+        reads += rr
+    rnd.shuffle(reads)
+    ix = S.Index.from_sequences(seqs, 12, 1000)
+    got = ix.align_reads(reads, 3)
+    want = orc.pipeline(orc.Index(seqs, 12, 1000), reads, 3)
+    assert got.n == len(want)
+    for i, w in enumerate(want):
+        g_rec = got.record(i)
+        for gk, wk in zip(REC_KEYS, ORC_KEYS):
+            assert g_rec[gk] == w[wk], (i, gk)
