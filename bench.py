@@ -107,6 +107,77 @@ def plain_pass(groups):
             C.c_void_p(b["rc"].ptr), n, None))
 
 
+def c2_leg(n_reads, steps, cpu_reads):
+    """Phase 1 + phase 2 over the C2 workload: 50 Mbp / 24 sequences, k=12,
+    tuple threshold 1000, maxHits 3.  Reports the device index build time,
+    phase-1 reads/s (both strands mapped) with its HBM figure, and reads/s of
+    the whole device pipeline (map -> 300-bp windows -> batched SW -> best-hit
+    choice), with the oracle's CPU pipeline timed on a sample beside it."""
+    from swift_amd import search, synth
+    from swift_amd._lib import check, lib
+    from swift_amd.device import DeviceBuffer, synchronize
+    t0 = time.perf_counter()
+    seqs, reads, _ = synth.c2_workload(n_reads=n_reads)
+    log(f"[bench] C2 workload: {sum(map(len, seqs)):.3e} bp, {len(reads)} reads "
+        f"({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    ix = search.Index.from_sequences(seqs, 12, 1000)
+    build_s = time.perf_counter() - t0
+    info = ix.info()
+    rb = search.ReadBatch(reads)
+    ns, mh = 2 * rb.n, 3
+    ref, sh, pos = DeviceBuffer(ns * mh), DeviceBuffer(4 * ns * mh), DeviceBuffer(4 * ns * mh)
+    nb, nbig, nh = (DeviceBuffer(4 * ns) for _ in range(3))
+
+    def map_once():
+        check(lib().swift_map_reads(ix._h, rb.seq.cptr, rb.off.cptr, rb.len.cptr, rb.n,
+                                    rb.max_len, mh, ref.cptr, sh.cptr, pos.cptr, nb.cptr,
+                                    nbig.cptr, nh.cptr, None))
+
+    map_once()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        map_once()
+    synchronize()
+    map_s = (time.perf_counter() - t0) / steps
+    hits = nh.download(np.int32, ns).astype(np.int64)
+    best = nb.download(np.int32, ns).astype(np.int64)
+    lens = np.repeat(np.array([len(r) for r in reads], np.int64), 2)
+    p1_bytes = float((8 * np.maximum(lens - 12 + 1, 0) + 5 * hits + 9 * best).sum())
+    out = search.RecordBuffers(2 * rb.n * mh, rb.max_len)
+    ix.align_batch(rb, out, mh)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ix.align_batch(rb, out, mh)
+    synchronize()
+    pipe_s = (time.perf_counter() - t0) / steps
+    recs = out.results(strings=False)
+    res = {"workload": "C2: 50 Mbp / 24 sequences (GC 41 %, 5 % tandem repeats), "
+                       f"{len(reads)} reads of 100 bp (5 % random), k=12, thr 1000, maxHits 3",
+           "index_build_s": round(build_s, 3), "kept_tuples": info["total"],
+           "phase1_reads_per_s": round(len(reads) / map_s, 1),
+           "phase1_ms": round(map_s * 1e3, 3),
+           "phase1_hbm_gbs": round(p1_bytes / map_s / 1e9, 2),
+           "pipeline_reads_per_s": round(len(reads) / pipe_s, 1),
+           "pipeline_ms": round(pipe_s * 1e3, 3), "records": recs.n,
+           "hits_per_strand": round(float(hits.mean()), 2)}
+    if cpu_reads > 0:
+        from oracle import oracle as O
+        sample = reads[:cpu_reads]
+        t0 = time.perf_counter()
+        oix = O.Index(seqs, 12, 1000)
+        t1 = time.perf_counter()
+        O.pipeline(oix, sample, mh)
+        dt = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": round(len(sample) / dt, 1), "unit": "reads/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"first {len(sample)} reads, oracle pipeline "
+                                         f"(index build {t1 - t0:.1f} s not included)"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,12 +188,34 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pairs", type=int, default=800000)
+    ap.add_argument("--c2-reads", type=int, default=100000)
+    ap.add_argument("--c2-cpu-reads", type=int, default=3000)
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--c2-only", action="store_true", help="only the C2 pipeline leg (no JSON "
+                    "bench line contract; for profiling)")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB 5 times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        n = args.calib_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": n, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the alignment "
               "hot path has no cross-GPU exchange; see DESIGN.md for how it shards.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.c2_only:
+        print(json.dumps(c2_leg(args.c2_reads, args.steps, 0)), flush=True)
+        return
 
     from swift_amd import align, synth
     from swift_amd._lib import lib
@@ -193,6 +286,9 @@ def main():
                                  "calls": len(groups)}
         out["speedup_vs_plain"] = round(gcups / (cells / pdt / 1e9), 3)
         del groups
+    if not args.no_c2:
+        out["c2"] = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads)
+        log(f"[bench] C2: {out['c2']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(qbuf, qoff, qlen, rbuf, roff, rlen, args.cpu_pairs)
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")

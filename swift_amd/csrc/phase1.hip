@@ -25,6 +25,7 @@
 #include <algorithm>
 
 #include "phase1.hpp"
+#include "scan.hpp"
 
 namespace swift {
 
@@ -493,37 +494,6 @@ __global__ void __launch_bounds__(256) map_strands_big(MapArgs a) {
 // ---------------------------------------------------------------------------
 // Records: per-strand prefix, hit -> record, region window per record
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) strand_prefix(const int32_t *nbest, int nstrands,
-                                                      int32_t *pair0, int32_t *npairs) {
-    __shared__ int32_t part[1024];
-    const int per = (nstrands + 1023) / 1024;
-    int32_t sum = 0;
-    for (int i = 0; i < per; ++i) {
-        int t = threadIdx.x * per + i;
-        if (t < nstrands) sum += nbest[t];
-    }
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int32_t run = part[threadIdx.x] - sum;
-    for (int i = 0; i < per; ++i) {
-        int t = threadIdx.x * per + i;
-        if (t < nstrands) {
-            pair0[t] = run;
-            run += nbest[t];
-        }
-    }
-    if (threadIdx.x == 1023) {
-        pair0[nstrands] = part[1023];
-        *npairs = part[1023];
-    }
-}
-
 __global__ void __launch_bounds__(256) hits_to_records(MapArgs m, PairsDev p) {
     const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
     if (sidx >= 2 * m.rd.nreads) return;
@@ -689,8 +659,10 @@ hipError_t launch_map(const MapArgs &a, hipStream_t st) {
 hipError_t launch_records(const MapArgs &m, const PairsDev &p, int before, int win,
                           hipStream_t st) {
     const int ns = 2 * m.rd.nreads;
-    hipLaunchKernelGGL(strand_prefix, dim3(1), dim3(1024), 0, st, m.out.nbest, ns, p.pair0,
-                       p.npairs);
+    hipError_t e = launch_scan<int32_t>(LoadArr<int32_t>{m.out.nbest}, StoreArr<int32_t>{p.pair0},
+                                        nullptr, ns, p.scan_sums, nullptr, p.pair0 + ns, p.npairs,
+                                        st);
+    if (e != hipSuccess) return e;
     if (ns > 0)
         hipLaunchKernelGGL(hits_to_records, dim3((ns + 255) / 256), dim3(256), 0, st, m, p);
     if (p.capacity > 0)

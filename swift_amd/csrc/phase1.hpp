@@ -91,6 +91,7 @@ struct PairsDev {
     int32_t *qlen, *rlen;
     int capacity;
     int64_t cstride;
+    int32_t *scan_sums;       // [2 * nreads / 2048 + 2]
 };
 
 hipError_t launch_index_build(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,

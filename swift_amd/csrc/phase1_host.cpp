@@ -296,7 +296,7 @@ static int index_export(const IndexObj &ix, int32_t *start, int32_t *count, int3
 // ---------------------------------------------------------------------------
 struct MapWork {
     DevArr ref, shift, pos, nbest, nbig, nhits, strand_seq, ovl_list, flags, scratch;
-    DevArr pair0, qoff, roff, coff, qlen, rlen;
+    DevArr pair0, qoff, roff, coff, qlen, rlen, scan_sums;
 };
 
 static std::mutex g_p1_mu;
@@ -427,6 +427,7 @@ static int align_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     if (rc) return rc;
     const size_t c = (size_t)std::max<int64_t>(cap, 1);
     if ((rc = ensure(w.pair0, 4 * (2 * (size_t)rd.nreads + 16)))) return rc;
+    if ((rc = ensure(w.scan_sums, 4 * (2 * (size_t)rd.nreads / 2048 + 16)))) return rc;
     if ((rc = ensure(w.qoff, 8 * c))) return rc;
     if ((rc = ensure(w.roff, 8 * c))) return rc;
     if ((rc = ensure(w.coff, 8 * c))) return rc;
@@ -450,6 +451,7 @@ static int align_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     p.rlen = w.rlen.as<int32_t>();
     p.capacity = (int)cap;
     p.cstride = rec.cons_stride;
+    p.scan_sums = w.scan_sums.as<int32_t>();
     SWIFT_HIP_TRY(launch_records(m, p, kBefore, kWin, st));
     BatchIO io;
     io.qseq = m.strand_seq;

@@ -69,7 +69,7 @@ struct Workspace {
     // plain path
     DevBuf p_trace, p_res, p_ptr;
     // batched path
-    DevBuf trace, res, ptr, bin_of, order, order2, tasks, task_words;
+    DevBuf trace, res, ptr, bin_of, order, order2, tasks, task_words, scan_sums;
     DevBuf bins;      // bin_count, bin_cursor (x2 passes)
     DevBuf segs;      // seg_task0 / seg_first / seg_len (x2 passes)
     DevBuf counters;  // int32 counters
@@ -218,6 +218,7 @@ static int reserve_common(Workspace &w, int npairs) {
     if ((rc = ensure_streams(w))) return rc;
     if ((rc = w.tasks.ensure((size_t)mt * sizeof(Task)))) return rc;
     if ((rc = w.task_words.ensure((size_t)mt * 8))) return rc;
+    if ((rc = w.scan_sums.ensure(((size_t)mt / 2048 + 2) * 8))) return rc;
     if ((rc = w.bins.ensure((size_t)4 * kNumBins * 4))) return rc;
     if ((rc = w.segs.ensure((size_t)2 * 3 * (2 * kRBins + 1) * 4))) return rc;
     if ((rc = w.counters.ensure(256 * 4))) return rc;
@@ -286,7 +287,8 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
     TaskArgs t1;
     pass_args(0, nullptr, force32 ? 1 : 0, b1, t1);
     SWIFT_HIP_TRY(launch_binning(b1, t1, w.order.as<int32_t>(), w.tasks.as<Task>(),
-                                 w.task_words.as<int64_t>(), nullptr, tot + 0, mt, st));
+                                 w.task_words.as<int64_t>(), nullptr, tot + 0,
+                                 w.scan_sums.as<int64_t>(), mt, st));
     int64_t words1 = 0;
     if (sync_sizes) {
         SWIFT_HIP_TRY(hipMemcpyAsync(&words1, tot + 0, 8, hipMemcpyDeviceToHost, st));
@@ -337,7 +339,8 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
         TaskArgs t2;
         pass_args(1, w.res.as<PairResult>(), 1, b2, t2);
         SWIFT_HIP_TRY(launch_binning(b2, t2, w.order2.as<int32_t>(), w.tasks.as<Task>(),
-                                     w.task_words.as<int64_t>(), tot + 0, tot + 1, mt, st));
+                                     w.task_words.as<int64_t>(), tot + 0, tot + 1,
+                                     w.scan_sums.as<int64_t>(), mt, st));
         int32_t nre = 0;
         if (sync_sizes) {
             SWIFT_HIP_TRY(hipMemcpyAsync(&words2, tot + 1, 8, hipMemcpyDeviceToHost, st));

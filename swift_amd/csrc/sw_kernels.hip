@@ -32,6 +32,7 @@
 //    back with one lane per pair.
 #include "sw_types.hpp"
 #include "sw_host.hpp"
+#include "scan.hpp"
 
 #include <type_traits>
 
@@ -653,16 +654,14 @@ sw_batch_kernel(BatchArgs a) {
 // Bins are (class, R, rlen/8) and are visited longest-first, so tasks are
 // served in decreasing cost (LPT) from the persistent work queue.
 // ---------------------------------------------------------------------------
-__global__ void sw_bin_classify(BinArgs a) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npairs) return;
+// pair -> bin (or -1 when the pair takes no part in this pass)
+__device__ __forceinline__ int classify_pair(const BinArgs &a, int p) {
     int ql = a.qlen[p], rl = a.rlen[p];
     if (ql < 0 || rl < 0 || ql > kMaxBatchQ || rl > kMaxBatchR) {
         // lengths beyond the batched kernels: the pair is left out (score 0)
         // and the call reports the error
-        a.bin_of[p] = -1;
         atomicOr(a.error, 1);
-        return;
+        return -1;
     }
     if (ql == 0 || rl == 0) {
         // all-zero matrix: start 0, end -1, length 0 (src/smithWaterman.cu
@@ -673,12 +672,11 @@ __global__ void sw_bin_classify(BinArgs a) {
             a.end_out[p] = -1;
             a.alen_out[p] = 0;
         }
-        a.bin_of[p] = -1;
-        return;
+        return -1;
     }
     int cls;
     if (a.prev) {
-        if (!(a.prev[p].flags & 1)) { a.bin_of[p] = -1; return; }
+        if (!(a.prev[p].flags & 1)) return -1;
         cls = 1;
     } else {
         // references with bytes outside ACGTNacgtn are caught by the int16
@@ -689,9 +687,32 @@ __global__ void sw_bin_classify(BinArgs a) {
     if (cls == 1) R = r32_class(R);
     int lb = min(kLenBins - 1, rl >> 3);
     // bin order: class, then R descending, then rlen descending
-    int bin = (cls * kRBins + (kRBins - R)) * kLenBins + (kLenBins - 1 - lb);
-    a.bin_of[p] = bin;
-    atomicAdd(&a.bin_count[bin], 1);
+    return (cls * kRBins + (kRBins - R)) * kLenBins + (kLenBins - 1 - lb);
+}
+
+// Pairs [blockIdx.x * kBinChunk, +kBinChunk) are counted in an LDS
+// histogram first, so a bin costs one global atomic per workgroup however
+// many pairs share it (uniform reads put every pair in one bin).
+constexpr int kBinPer = 8;
+constexpr int kBinChunk = 256 * kBinPer;
+
+__global__ void __launch_bounds__(256) sw_bin_classify(BinArgs a) {
+    __shared__ int32_t hist[kNumBins];
+    for (int b = threadIdx.x; b < kNumBins; b += 256) hist[b] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kBinChunk;
+#pragma unroll
+    for (int i = 0; i < kBinPer; ++i) {
+        int p = base + i * 256 + (int)threadIdx.x;
+        if (p < a.npairs) {
+            int b = classify_pair(a, p);
+            a.bin_of[p] = b;
+            if (b >= 0) atomicAdd(&hist[b], 1);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kNumBins; b += 256)
+        if (hist[b]) atomicAdd(&a.bin_count[b], hist[b]);
 }
 
 // single workgroup: bin prefix sums and per-(class, R) task counts
@@ -722,18 +743,23 @@ __global__ void __launch_bounds__(1024) sw_bin_scan(TaskArgs a) {
         run += loc[i];
     }
     __syncthreads();
+    // segment (class, R) lengths in parallel, then their task offsets
+    __shared__ int32_t seg_tasks[2 * kRBins];
+    if (tid < 2 * kRBins) {
+        const int seg = tid, cls = seg / kRBins;
+        int32_t len = 0;
+        for (int lb = 0; lb < kLenBins; ++lb) len += a.bin_count[seg * kLenBins + lb];
+        const int slots = cls == 0 ? kSlots16 : kSlots32;
+        a.seg_first[seg] = a.bin_cursor[seg * kLenBins];
+        a.seg_len[seg] = len;
+        seg_tasks[seg] = (len + slots - 1) / slots;
+    }
+    __syncthreads();
     if (tid == 0) {
         int32_t t0 = 0;
         for (int seg = 0; seg < 2 * kRBins; ++seg) {
-            int cls = seg / kRBins;
-            int32_t first = a.bin_cursor[seg * kLenBins];
-            int32_t len = 0;
-            for (int lb = 0; lb < kLenBins; ++lb) len += a.bin_count[seg * kLenBins + lb];
-            int slots = cls == 0 ? 8 : 4;
             a.seg_task0[seg] = t0;
-            a.seg_first[seg] = first;
-            a.seg_len[seg] = len;
-            t0 += (len + slots - 1) / slots;
+            t0 += seg_tasks[seg];
         }
         a.seg_task0[2 * kRBins] = t0;
         *a.ntasks = t0;
@@ -741,14 +767,28 @@ __global__ void __launch_bounds__(1024) sw_bin_scan(TaskArgs a) {
     }
 }
 
-__global__ void sw_bin_scatter(const int32_t *bin_of, int npairs, int32_t *bin_cursor,
-                               int32_t *order) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
-    int b = bin_of[p];
-    if (b < 0) return;
-    int pos = atomicAdd(&bin_cursor[b], 1);
-    order[pos] = p;
+// Same chunks as sw_bin_classify: ranks within the chunk from LDS atomics,
+// one global cursor reservation per (chunk, bin).
+__global__ void __launch_bounds__(256) sw_bin_scatter(const int32_t *bin_of, int npairs,
+                                                      int32_t *bin_cursor, int32_t *order) {
+    __shared__ int32_t hist[kNumBins];
+    for (int b = threadIdx.x; b < kNumBins; b += 256) hist[b] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kBinChunk;
+    int bin[kBinPer], rank[kBinPer];
+#pragma unroll
+    for (int i = 0; i < kBinPer; ++i) {
+        int p = base + i * 256 + (int)threadIdx.x;
+        bin[i] = p < npairs ? bin_of[p] : -1;
+        rank[i] = bin[i] >= 0 ? atomicAdd(&hist[bin[i]], 1) : 0;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kNumBins; b += 256)
+        if (hist[b]) hist[b] = atomicAdd(&bin_cursor[b], hist[b]);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kBinPer; ++i)
+        if (bin[i] >= 0) order[hist[bin[i]] + rank[i]] = base + i * 256 + (int)threadIdx.x;
 }
 
 // task descriptors (trace sizes filled, bases scanned afterwards)
@@ -786,37 +826,10 @@ __global__ void sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_firs
     task_words[t] = (int64_t)((tk.steps + 3) & ~3) * W * 64;  // whole 4-step granules
 }
 
-// single workgroup exclusive scan of task trace sizes into the descriptors
-__global__ void __launch_bounds__(1024)
-sw_task_bases(Task *tasks, const int64_t *task_words, const int32_t *ntasks_p,
-              const int64_t *base0, int64_t *total_words) {
-    __shared__ int64_t part[1024];
-    const int tid = threadIdx.x;
-    const int n = *ntasks_p;
-    const int per = (n + 1023) / 1024;
-    int64_t s = 0;
-    for (int i = 0; i < per; ++i) {
-        int t = tid * per + i;
-        if (t < n) s += task_words[t];
-    }
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int64_t v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    int64_t run = part[tid] - s + (base0 ? *base0 : 0);
-    for (int i = 0; i < per; ++i) {
-        int t = tid * per + i;
-        if (t < n) {
-            tasks[t].trace_base = run;
-            run += task_words[t];
-        }
-    }
-    if (tid == 1023) *total_words = part[1023] + (base0 ? *base0 : 0);
-}
+struct StoreTaskBase {
+    Task *t;
+    __device__ void operator()(int i, int64_t v) const { t[i].trace_base = v; }
+};
 
 __global__ void fill_i32(int32_t *p, int n, int32_t v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -913,25 +926,23 @@ hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count,
 
 hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order, Task *tasks,
                           int64_t *task_words, const int64_t *base0, int64_t *total_words,
-                          int max_tasks,
-                          hipStream_t st) {
+                          int64_t *scan_sums, int max_tasks, hipStream_t st) {
     int n = bin.npairs;
     hipLaunchKernelGGL(fill_i32, dim3((kNumBins + 255) / 256), dim3(256), 0, st, bin.bin_count,
                        kNumBins, 0);
-    if (n > 0)
-        hipLaunchKernelGGL(sw_bin_classify, dim3((n + 255) / 256), dim3(256), 0, st, bin);
+    const int chunks = (n + kBinChunk - 1) / kBinChunk;
+    if (n > 0) hipLaunchKernelGGL(sw_bin_classify, dim3(chunks), dim3(256), 0, st, bin);
     hipLaunchKernelGGL(sw_bin_scan, dim3(1), dim3(1024), 0, st, ta);
     if (n > 0)
-        hipLaunchKernelGGL(sw_bin_scatter, dim3((n + 255) / 256), dim3(256), 0, st, bin.bin_of,
-                           n, ta.bin_cursor, order);
+        hipLaunchKernelGGL(sw_bin_scatter, dim3(chunks), dim3(256), 0, st, bin.bin_of, n,
+                           ta.bin_cursor, order);
     if (max_tasks > 0) {
         hipLaunchKernelGGL(sw_build_tasks, dim3((max_tasks + 255) / 256), dim3(256), 0, st,
                            ta.seg_task0, ta.seg_first, ta.seg_len, ta.ntasks, order, bin.rlen,
                            tasks, task_words);
     }
-    hipLaunchKernelGGL(sw_task_bases, dim3(1), dim3(1024), 0, st, tasks, task_words, ta.ntasks,
-                       base0, total_words);
-    return hipGetLastError();
+    return launch_scan<int64_t>(LoadArr<int64_t>{task_words}, StoreTaskBase{tasks}, ta.ntasks,
+                                max_tasks, scan_sums, base0, total_words, nullptr, st);
 }
 
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st) {

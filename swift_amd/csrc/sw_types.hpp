@@ -119,8 +119,9 @@ struct TaskArgs {
 // launch helpers (sw_kernels.hip)
 hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
                         hipStream_t st);
+// scan_sums: room for max_tasks / 2048 + 1 int64 values
 hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order, Task *tasks,
                           int64_t *task_words, const int64_t *base0, int64_t *total_words,
-                          int max_tasks, hipStream_t st);
+                          int64_t *scan_sums, int max_tasks, hipStream_t st);
 
 }  // namespace swift
