@@ -32,6 +32,11 @@ METRIC = "SW GCUPS + reads/s at 1 MI355X, batched vs per-pair; p50/p99 latency"
 CELL_OPS = 16                 # algorithmic integer ops per DP cell
 PEAK_INT16_TOPS = 78.6        # 256 CU x 64 lanes x 2 (packed) x 2.4 GHz
 PEAK_INT32_TOPS = 39.3
+# HBM bytes per DP cell of one swift_sw_batch forward pass on this workload,
+# from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) counts, each scaled by
+# the same counter's reading for a device copy of known size
+# (profiles/sw_batch_c3_pmc.txt)
+TRAFFIC_B_PER_CELL = 0.6404
 
 
 def log(*a):
@@ -264,7 +269,10 @@ def main():
                    "lanes": "packed int16 with int32 re-run"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": None, "kernel": "sw_batch_kernel<R,int16> (all row classes)",
+                     "traffic": round(TRAFFIC_B_PER_CELL * cells) if args.pairs == 1_000_000
+                     else None,
+                     "traffic_unit": "bytes per forward pass (PMC, calibrated)",
+                     "kernel": "sw_batch_kernel<R,int16> (all row classes)",
                      "ops_per_cell": CELL_OPS, "forward_ms": round(fwd_ms, 3)},
         "phases_ms": {"binning": round(float(np.mean(binning)), 3),
                       "forward": round(fwd_ms, 3), "int32_rerun": round(float(np.mean(rerun)), 3),
