@@ -112,7 +112,7 @@ def plain_pass(groups):
             C.c_void_p(b["rc"].ptr), n, None))
 
 
-def c2_leg(n_reads, steps, cpu_reads):
+def c2_leg(n_reads, steps, cpu_reads, keep=None):
     """Phase 1 + phase 2 over the C2 workload: 50 Mbp / 24 sequences, k=12,
     tuple threshold 1000, maxHits 3.  Reports the device index build time,
     phase-1 reads/s (both strands mapped) with its HBM figure, and reads/s of
@@ -159,6 +159,8 @@ def c2_leg(n_reads, steps, cpu_reads):
     synchronize()
     pipe_s = (time.perf_counter() - t0) / steps
     recs = out.results(strings=False)
+    if keep is not None:
+        keep["index"], keep["seqs"] = ix, seqs
     res = {"workload": "C2: 50 Mbp / 24 sequences (GC 41 %, 5 % tandem repeats), "
                        f"{len(reads)} reads of 100 bp (5 % random), k=12, thr 1000, maxHits 3",
            "index_build_s": round(build_s, 3), "kept_tuples": info["total"],
@@ -183,6 +185,83 @@ def c2_leg(n_reads, steps, cpu_reads):
     return res
 
 
+def c5_leg(index, seqs, n_requests, seed=20261205):
+    """Serving: read batches of log-uniform size 1..4096 (lengths U{50..300},
+    1.5 % error) submitted to a swift_server with the index resident.  First
+    every request at once (capacity, reads/s), then Poisson arrivals at 70 %
+    of that capacity; latency = submit -> records ready, measured by the
+    server."""
+    import threading
+    from swift_amd import synth
+    from swift_amd.serve import Server
+    rng = np.random.default_rng(seed)
+    arr = [np.frombuffer(s, np.uint8) if isinstance(s, bytes) else s for s in seqs]
+    pool = []
+    for L in range(50, 301, 10):
+        rr, _ = synth.sample_reads(rng, arr, 600, L, 0.015, 0.0015)
+        pool += rr
+    order = rng.permutation(len(pool))
+    pool = [pool[i] for i in order]
+    sizes = np.exp(rng.uniform(0, np.log(4096), n_requests)).astype(int)
+    reqs = []
+    for n in sizes:
+        o = int(rng.integers(0, len(pool) - n))
+        reqs.append(pool[o:o + n])
+    total_reads = int(sizes.sum())
+    log(f"[bench] C5: {n_requests} requests, {total_reads} reads (pool {len(pool)})")
+    res = {"workload": f"C5 serving: {n_requests} requests of log-uniform 1..4096 reads, "
+                       "lengths U{50..300}, 1.5 % error, index resident",
+           "requests": n_requests, "reads": total_reads}
+    packed = [Server.pack(r) for r in reqs]
+    with Server(index, nslots=4, mem_budget=48 << 30, max_batch_reads=8192) as srv:
+        for p in packed[:8]:  # warm the slots
+            srv.result(srv.submit_packed(*p), strings=False)
+        t0 = time.perf_counter()
+        ids = [srv.submit_packed(*p) for p in packed]
+        for i in ids:
+            srv.result(i, strings=False)
+        cap_s = time.perf_counter() - t0
+        capacity = total_reads / cap_s
+        lam = 0.7 * capacity
+        mean_size = total_reads / n_requests
+        rate = lam / mean_size                      # requests per second
+        gaps = rng.exponential(1.0 / rate, n_requests)
+        lat = [0.0] * n_requests
+        ids = [None] * n_requests
+        ready = threading.Semaphore(0)
+
+        def collect():
+            for k in range(n_requests):
+                ready.acquire()
+                _, lat[k] = srv.result(ids[k], strings=False)
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+        due = t0
+        for k in range(n_requests):
+            due += gaps[k]
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids[k] = srv.submit_packed(*packed[k])
+            ready.release()
+        th.join()
+        run_s = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array(lat)
+    res.update({"capacity_reads_per_s": round(capacity, 1),
+                "arrival_rate_reads_per_s": round(lam, 1),
+                "arrival_rate_requests_per_s": round(rate, 2),
+                "served_reads_per_s": round(total_reads / run_s, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_ms": round(float(lat.mean()), 3),
+                "batches": st["batches"], "max_coalesced": st["max_coalesced"],
+                "slots": 4})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +275,10 @@ def main():
     ap.add_argument("--c2-reads", type=int, default=100000)
     ap.add_argument("--c2-cpu-reads", type=int, default=3000)
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-requests", type=int, default=400)
+    ap.add_argument("--c5-only", action="store_true", help="only the C5 serving leg on the C2 "
+                    "index (for profiling)")
     ap.add_argument("--c2-only", action="store_true", help="only the C2 pipeline leg (no JSON "
                     "bench line contract; for profiling)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -218,8 +301,18 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.c5_only:
+        from swift_amd import search, synth
+        seqs, _, _ = synth.c2_workload(n_reads=10)
+        ix = search.Index.from_sequences(seqs, 12, 1000)
+        print(json.dumps(c5_leg(ix, seqs, args.c5_requests)), flush=True)
+        return
     if args.c2_only:
-        print(json.dumps(c2_leg(args.c2_reads, args.steps, 0)), flush=True)
+        keep = {}
+        res = c2_leg(args.c2_reads, args.steps, 0, keep)
+        if not args.no_c5:
+            res["c5"] = c5_leg(keep["index"], keep["seqs"], args.c5_requests)
+        print(json.dumps(res), flush=True)
         return
 
     from swift_amd import align, synth
@@ -295,8 +388,15 @@ def main():
         out["speedup_vs_plain"] = round(gcups / (cells / pdt / 1e9), 3)
         del groups
     if not args.no_c2:
-        out["c2"] = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads)
+        keep = {}
+        out["c2"] = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads,
+                           keep)
         log(f"[bench] C2: {out['c2']}")
+        if not args.no_c5:
+            out["c5"] = c5_leg(keep["index"], keep["seqs"], args.c5_requests)
+            out["c5"]["index"] = "C2 index (50 Mbp / 24 sequences, k=12) resident"
+            log(f"[bench] C5: {out['c5']}")
+        del keep
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(qbuf, qoff, qlen, rbuf, roff, rlen, args.cpu_pairs)
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")

@@ -36,7 +36,8 @@ enum {
     SWIFT_ERR_HIP = 2,
     SWIFT_ERR_NO_INDEX = 3,
     SWIFT_ERR_IO = 4,
-    SWIFT_ERR_CAPACITY = 5
+    SWIFT_ERR_CAPACITY = 5,
+    SWIFT_ERR_TIMEOUT = 6
 };
 
 const char *swift_last_error(void);
@@ -230,6 +231,38 @@ int swift_align_reads(const swift_index *ix, const char *reads_d, const int64_t 
                       const int32_t *len_d, int nreads, int max_read_len, int max_hits,
                       int match, int mismatch, int gap_open, int gap_ext,
                       const swift_records *rec, void *stream);
+
+/* ======================================================== serving loop
+ * No reference counterpart (the reference is a batch command-line tool).
+ * A server keeps an index resident and aligns read batches submitted from
+ * any host thread.  Each of nslots slots (a host thread with its own stream
+ * and scratch) takes queued requests in arrival order and coalesces them
+ * into one device batch while the batch's device-memory estimate stays
+ * within mem_budget_bytes / nslots and max_batch_reads; slots run
+ * concurrently, so batches admitted while others are on the GPU overlap
+ * there.  A request's records are the same whatever it is batched with:
+ * swift_align_reads over the request alone gives identical records. */
+typedef struct swift_server swift_server;
+
+int swift_server_create(const swift_index *ix, int nslots, int64_t mem_budget_bytes,
+                        int max_batch_reads, int max_read_len, int max_hits, int match,
+                        int mismatch, int gap_open, int gap_ext, swift_server **out);
+/* Copies the reads (host memory) and queues them; *id names the request.
+ * Requests larger than one slot are rejected (SWIFT_ERR_CAPACITY). */
+int swift_server_submit(swift_server *s, const char *reads, const int64_t *off, const int32_t *len,
+                        int nreads, uint64_t *id);
+/* Waits up to timeout_ms (< 0: no limit); *nrecords and the submit-to-done
+ * latency in ms on success, SWIFT_ERR_TIMEOUT when still running. */
+int swift_server_wait(swift_server *s, uint64_t id, int timeout_ms, int *nrecords,
+                      double *latency_ms);
+/* Copies a finished request's records into host arrays (swift_records with
+ * host pointers; NULL fields are skipped; read indices are relative to the
+ * request) and releases the request. */
+int swift_server_fetch(swift_server *s, uint64_t id, const swift_records *out);
+int swift_server_stats(swift_server *s, int64_t *completed, int64_t *batches,
+                       int64_t *batched_reads, int64_t *max_coalesced, int64_t *device_bytes);
+/* Stops the slots (queued requests are dropped) and frees the server. */
+int swift_server_destroy(swift_server *s);
 
 #ifdef __cplusplus
 }

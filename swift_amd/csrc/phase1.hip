@@ -569,6 +569,41 @@ __global__ void __launch_bounds__(256) choose_best(PairsDev p, int nreads, const
 }
 
 // ---------------------------------------------------------------------------
+// Consensus strings of records [0, *n) packed back to back (record i's
+// alen[i] bytes at pos[i], an exclusive prefix of alen): one wave per record.
+// ---------------------------------------------------------------------------
+struct LoadLen64 {
+    const int32_t *p;
+    __device__ int64_t operator()(int i) const { return p[i]; }
+};
+
+__global__ void __launch_bounds__(256) pack_cons(const int32_t *n_dev, const int32_t *alen,
+                                                 const int64_t *pos, const char *qc,
+                                                 const char *rc, int64_t cs, char *oq, char *orr) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= *n_dev) return;
+    const int lane = threadIdx.x & 63;
+    const int L = alen[i];
+    const char *q = qc + (int64_t)i * cs, *r = rc + (int64_t)i * cs;
+    char *dq = oq + pos[i], *dr = orr + pos[i];
+    for (int k = lane; k < L; k += 64) {
+        dq[k] = q[k];
+        dr[k] = r[k];
+    }
+}
+
+hipError_t launch_pack_cons(const int32_t *n_dev, int cap, const int32_t *alen, int64_t *pos,
+                            int64_t *sums, int64_t *total, const char *qc, const char *rc,
+                            int64_t cs, char *oq, char *orr, hipStream_t st) {
+    hipError_t e = launch_scan<int64_t>(LoadLen64{alen}, StoreArr<int64_t>{pos}, n_dev, cap, sums,
+                                        nullptr, total, nullptr, st);
+    if (e != hipSuccess || cap <= 0) return e;
+    hipLaunchKernelGGL(pack_cons, dim3((cap + 3) / 4), dim3(256), 0, st, n_dev, alen, pos, qc, rc,
+                       cs, oq, orr);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // mapHits6GetBestHits over caller hits (drop-in; one workgroup)
 // ---------------------------------------------------------------------------
 struct HitTriple {

@@ -103,6 +103,12 @@ hipError_t launch_records(const MapArgs &m, const PairsDev &p, int before, int w
                           hipStream_t st);
 hipError_t launch_choose_best(const PairsDev &p, int nreads, const int32_t *score,
                               int32_t *keep, int32_t *ntied, hipStream_t st);
+// Packs records' consensus strings (stride cs) back to back: pos[i] =
+// exclusive prefix of alen over [0, *n_dev), *total = sum; sums holds
+// cap / 2048 + 2 values.
+hipError_t launch_pack_cons(const int32_t *n_dev, int cap, const int32_t *alen, int64_t *pos,
+                            int64_t *sums, int64_t *total, const char *qc, const char *rc,
+                            int64_t cs, char *oq, char *orr, hipStream_t st);
 // mapHits6GetBestHits over caller hits already on the device: sorts the
 // triples in place (refIdx, shift, refPos ascending) and writes the best
 // clusters' first hits.  kbuf holds next_pow2(n) 16-byte keys.

@@ -11,34 +11,10 @@
 #include <string>
 #include <vector>
 
-#include "../../include/swift_abi.h"
-#include "phase1.hpp"
-#include "sw_host.hpp"
+#include "phase1_host.hpp"
 #include "sw_types.hpp"
 
 namespace swift {
-
-// Exact-size device array, freed with its owner.
-struct DevArr {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    int alloc(size_t b) {
-        release();
-        SWIFT_HIP_TRY(hipMalloc(&ptr, b ? b : 1));
-        bytes = b;
-        return kOk;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(ptr); }
-    DevArr() = default;
-    DevArr(const DevArr &) = delete;
-    DevArr &operator=(const DevArr &) = delete;
-    ~DevArr() { release(); }
-};
 
 // ---------------------------------------------------------------------------
 // FASTA readers
@@ -145,33 +121,6 @@ static int read_query_fasta(const char *path, QueryFile *q) {
 // ---------------------------------------------------------------------------
 // Index
 // ---------------------------------------------------------------------------
-struct IndexObj {
-    int k = 0, thr = 0, nseq = 0, max_actual = 0;
-    int64_t nhash = 0, ntup = 0, total = 0;
-    DevArr seq, seq_off, seq_len, tup0, count, start, actual, cursor, bucket, entry;
-    std::vector<int64_t> h_off, h_len;
-    std::vector<std::string> names;
-
-    IndexDev dev() const {
-        IndexDev d;
-        d.seq = seq.as<const char>();
-        d.seq_off = seq_off.as<const int64_t>();
-        d.seq_len = seq_len.as<const int64_t>();
-        d.tup0 = tup0.as<const int64_t>();
-        d.nseq = nseq;
-        d.k = k;
-        d.thr = thr;
-        d.nhash = nhash;
-        d.count = count.as<int32_t>();
-        d.start = start.as<int32_t>();
-        d.actual = actual.as<int32_t>();
-        d.cursor = cursor.as<int32_t>();
-        d.bucket = bucket.as<int2>();
-        d.entry = entry.as<uint64_t>();
-        return d;
-    }
-};
-
 static int64_t next_pow2_64(int64_t n) {
     int64_t p = 1;
     while (p < n) p <<= 1;
@@ -294,18 +243,17 @@ static int index_export(const IndexObj &ix, int32_t *start, int32_t *count, int3
 // ---------------------------------------------------------------------------
 // Mapping
 // ---------------------------------------------------------------------------
-struct MapWork {
-    DevArr ref, shift, pos, nbest, nbig, nhits, strand_seq, ovl_list, flags, scratch;
-    DevArr pair0, qoff, roff, coff, qlen, rlen, scan_sums;
-};
-
 static std::mutex g_p1_mu;
 static MapWork &map_work() {
     static MapWork *w = new MapWork;  // lives until the process ends
     return *w;
 }
 
-static int ensure(DevArr &a, size_t bytes) { return bytes <= a.bytes ? (int)kOk : a.alloc(bytes); }
+// grows by half again, so batches of varying size settle after a few calls
+// (hipFree synchronises the device)
+int ensure(DevArr &a, size_t bytes) {
+    return bytes <= a.bytes ? (int)kOk : a.alloc(bytes + bytes / 2 + 256);
+}
 
 // Fills `m` for a launch over `rd`; out arrays may come from the caller
 // (any of them null: the workspace's are used).
@@ -403,14 +351,29 @@ static int map_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len, i
 }
 
 // Phase 1 -> phase 2.
+constexpr int kBefore = 100, kWin = 300;  // src/align.cu:2388,2435
+
+int64_t align_bytes_estimate(int nreads, int max_read_len, int max_hits) {
+    const int64_t ns = 2 * (int64_t)nreads, cap = ns * std::max(max_hits, 0);
+    const int64_t qstride = (max_read_len + 15) & ~15;
+    const int64_t cstride = max_read_len + kWin + 2;
+    return ns * (std::max(max_hits, 1) * 9 + 16 + qstride) + cap * (13 * 4 + 8 + 2 * cstride) +
+           cap * 36 + batch_bytes_estimate((int)std::max<int64_t>(cap, 1), max_read_len, kWin);
+}
+
 static int align_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len, int max_hits,
                        const Scoring &sc, const swift_records &rec, hipStream_t st) {
     std::lock_guard<std::mutex> lk(g_p1_mu);
+    return align_reads_in(map_work(), nullptr, ix, rd, max_read_len, max_hits, sc, rec, st);
+}
+
+int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDev &rd,
+                   int max_read_len, int max_hits, const Scoring &sc, const swift_records &rec,
+                   hipStream_t st) {
     if (max_read_len > kMaxBatchQ) {
         set_error("the pipeline aligns reads up to %d bases (got %d)", kMaxBatchQ, max_read_len);
         return kErrCapacity;
     }
-    constexpr int kBefore = 100, kWin = 300;
     const int64_t cap = 2 * (int64_t)rd.nreads * std::max(max_hits, 0);
     if (cap > rec.capacity || cap >= INT32_MAX) {
         set_error("record capacity %d below 2 * nreads * max_hits = %lld", rec.capacity,
@@ -421,7 +384,6 @@ static int align_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
         set_error("cons_stride %lld below max_read_len + 302", (long long)rec.cons_stride);
         return kErrInvalid;
     }
-    MapWork &w = map_work();
     MapArgs m;
     int rc = prepare_map(ix, rd, max_read_len, max_hits, &m, w, true);
     if (rc) return rc;
@@ -470,7 +432,7 @@ static int align_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     io.qcons = rec.qry_cons;
     io.rcons = rec.ref_cons;
     io.cons_off = p.coff;
-    if ((rc = sw_batch_run(io, sc, st, true))) return rc;
+    if ((rc = sws ? sw_batch_run_in(sws, io, sc, st) : sw_batch_run(io, sc, st, true))) return rc;
     SWIFT_HIP_TRY(launch_choose_best(p, rd.nreads, rec.score, rec.keep, rec.ntied, st));
     int32_t flags[2];
     SWIFT_HIP_TRY(hipMemcpyAsync(flags, m.ovl_count, 8, hipMemcpyDeviceToHost, st));
@@ -506,10 +468,6 @@ static void die(const char *fn) {
 }  // namespace swift
 
 using namespace swift;
-
-struct swift_index {
-    IndexObj ix;
-};
 
 extern "C" {
 

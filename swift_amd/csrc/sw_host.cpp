@@ -53,8 +53,8 @@ struct Timing {
 };
 static Timing g_timing;
 
-static hipError_t mark(int i, hipStream_t st) {
-    if (!g_timing.on) return hipSuccess;
+static hipError_t mark(int i, hipStream_t st, bool timed) {
+    if (!g_timing.on || !timed) return hipSuccess;
     if (!g_timing.ev[i]) {
         hipError_t e = hipEventCreate(&g_timing.ev[i]);
         if (e != hipSuccess) return e;
@@ -81,10 +81,25 @@ struct Workspace {
     hipStream_t side[kSide] = {};
     hipEvent_t fork = nullptr, join[kSide] = {};
     bool streams_ready = false;
+    bool timed = false;       // records the HIP-event phase marks (the shared workspace)
+    std::mutex mu;            // one batched run at a time per workspace
+
+    DevBuf *all_bufs[17] = {&p_trace, &p_res, &p_ptr, &trace, &res, &ptr, &bin_of, &order,
+                            &order2, &tasks, &task_words, &scan_sums, &bins, &segs, &counters,
+                            &totals, nullptr};
 };
+
+static void init_workspace(Workspace *w) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) w->cu_count = prop.multiProcessorCount;
+    if (w->cu_count <= 0) w->cu_count = 256;
+}
 
 static std::mutex g_mu;
 
+// the workspace shared by the swift_sw_batch / alignSequences entry points
 static Workspace &ws() {
     static std::vector<Workspace *> all;
     int dev = 0;
@@ -92,11 +107,40 @@ static Workspace &ws() {
     if ((int)all.size() <= dev) all.resize(dev + 1, nullptr);
     if (!all[dev]) {
         all[dev] = new Workspace();
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) all[dev]->cu_count = prop.multiProcessorCount;
-        if (all[dev]->cu_count <= 0) all[dev]->cu_count = 256;
+        init_workspace(all[dev]);
+        all[dev]->timed = true;
     }
     return *all[dev];
+}
+
+// private workspaces (one per serving slot)
+Workspace *workspace_new() {
+    Workspace *w = new Workspace();
+    init_workspace(w);
+    return w;
+}
+
+void workspace_free(Workspace *w) {
+    if (!w) return;
+    for (int i = 0; w->all_bufs[i]; ++i) {
+        if (w->all_bufs[i]->ptr) (void)hipFree(w->all_bufs[i]->ptr);
+        w->all_bufs[i]->ptr = nullptr;
+        w->all_bufs[i]->bytes = 0;
+    }
+    if (w->streams_ready) {
+        for (int i = 0; i < Workspace::kSide; ++i) {
+            (void)hipStreamDestroy(w->side[i]);
+            (void)hipEventDestroy(w->join[i]);
+        }
+        (void)hipEventDestroy(w->fork);
+    }
+    delete w;
+}
+
+int64_t workspace_bytes(const Workspace *w) {
+    int64_t b = 0;
+    for (int i = 0; w->all_bufs[i]; ++i) b += (int64_t)w->all_bufs[i]->bytes;
+    return b;
 }
 
 BatchStats last_batch_stats() { return ws().stats; }
@@ -226,6 +270,14 @@ static int reserve_common(Workspace &w, int npairs) {
     return kOk;
 }
 
+// device bytes a batched run of this shape may hold (workspace, worst case)
+int64_t batch_bytes_estimate(int npairs, int maxq, int maxr) {
+    const int64_t n = std::max(npairs, 1), mt = max_tasks_for(npairs);
+    return n * (int64_t)(sizeof(PairResult) + sizeof(PairTrace) + 12) +
+           mt * (int64_t)(sizeof(Task) + 8) + 4 * (int64_t)worst_trace_words(npairs, maxq, maxr) +
+           (int64_t)4 * kNumBins * 4 + (1 << 16);
+}
+
 int reserve_batch(int npairs, int maxq, int maxr) {
     std::lock_guard<std::mutex> lk(g_mu);
     Workspace &w = ws();
@@ -239,8 +291,21 @@ int reserve_batch(int npairs, int maxq, int maxr) {
 constexpr int kCntError = 8;
 constexpr int kCntQueues = 16;
 constexpr int kQueuesPerPass = 2 * kRBins;
+static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStream_t st,
+                     bool sync_sizes);
+
 int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync_sizes) {
     std::lock_guard<std::mutex> lk(g_mu);
+    return batch_run(ws(), io, sc, st, sync_sizes);
+}
+
+int sw_batch_run_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(w->mu);
+    return batch_run(*w, io, sc, st, true);
+}
+
+static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStream_t st,
+                     bool sync_sizes) {
     if (io.npairs < 0) {
         set_error("negative pair count");
         return kErrInvalid;
@@ -254,7 +319,6 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
     int rc = validate_scoring(sc, io.maxq, io.maxr, &force32);
     if (rc) return rc;
     if (!tuning().int16) force32 = true;
-    Workspace &w = ws();
     w.stats = BatchStats{};
     if (io.npairs == 0) return kOk;
     if ((rc = reserve_common(w, io.npairs))) return rc;
@@ -281,7 +345,7 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
     // results start at "score 0": a pair no kernel reaches never walks
     SWIFT_HIP_TRY(launch_fill(w.res.as<int32_t>(), n * (int)(sizeof(PairResult) / 4), 0, st));
     SWIFT_HIP_TRY(launch_fill(cnt + kCntError, 1, 0, st));
-    SWIFT_HIP_TRY(mark(0, st));
+    SWIFT_HIP_TRY(mark(0, st, w.timed));
     // ---- pass 1: bin every pair
     BinArgs b1;
     TaskArgs t1;
@@ -290,10 +354,22 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
                                  w.task_words.as<int64_t>(), nullptr, tot + 0,
                                  w.scan_sums.as<int64_t>(), mt, st));
     int64_t words1 = 0;
+    int32_t seg1[2 * kRBins + 1], seg2[2 * kRBins + 1], cnt_h[16];
     if (sync_sizes) {
+        // trace size, per-class task counts (only non-empty classes are
+        // launched) and the error flag in one round trip
         SWIFT_HIP_TRY(hipMemcpyAsync(&words1, tot + 0, 8, hipMemcpyDeviceToHost, st));
+        SWIFT_HIP_TRY(hipMemcpyAsync(seg1, t1.seg_task0, sizeof(seg1), hipMemcpyDeviceToHost, st));
+        SWIFT_HIP_TRY(hipMemcpyAsync(cnt_h, cnt, sizeof(cnt_h), hipMemcpyDeviceToHost, st));
         SWIFT_HIP_TRY(hipStreamSynchronize(st));
+        if (cnt_h[kCntError]) {
+            set_error("pair lengths beyond maxq/maxr or the batched limits (%d/%d)", kMaxBatchQ,
+                      kMaxBatchR);
+            return kErrCapacity;
+        }
         if ((rc = w.trace.ensure((size_t)words1 * 4 + 4))) return rc;
+        w.stats.ntasks16 = force32 ? 0 : seg1[kRBins];
+        w.stats.ntasks32 = seg1[2 * kRBins] - seg1[kRBins];
     }
     SWIFT_HIP_TRY(launch_fill(cnt + kCntQueues, 2 * kQueuesPerPass, 0, st));
     BatchArgs ba{io.qseq, io.qoff, io.qlen, io.rseq, io.roff, io.rlen, w.order.as<int32_t>(),
@@ -304,34 +380,47 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
     // range from the device (empty segments exit at once), spread longest
     // rows first over the side streams so the classes run concurrently and
     // one class's tail overlaps the next class's work.
-    auto launch_classes = [&](const TaskArgs &t, bool with16, int pass) -> int {
-        SWIFT_HIP_TRY(hipEventRecord(w.fork, st));
-        for (int i = 0; i < Workspace::kSide; ++i)
-            SWIFT_HIP_TRY(hipStreamWaitEvent(w.side[i], w.fork, 0));
-        int k = 0;
+    // seg_host (task counts read back) skips empty classes; without it every
+    // class kernel is launched and empty ones exit at once.
+    auto launch_classes = [&](const TaskArgs &t, bool with16, int pass,
+                              const int32_t *seg_host) -> int {
+        int live[2 * kRBins], nlive = 0;
         for (int cls = with16 ? 0 : 1; cls < 2; ++cls) {
             for (int R = kRBins; R >= 1; --R) {
                 if (cls == 1 && r32_class(R) != R) continue;
                 int seg = cls * kRBins + (kRBins - R);
+                if (seg_host && seg_host[seg + 1] == seg_host[seg]) continue;
+                live[nlive++] = seg;
+            }
+        }
+        if (nlive == 0) return kOk;
+        const int nside = std::min(nlive, (int)Workspace::kSide);
+        SWIFT_HIP_TRY(hipEventRecord(w.fork, st));
+        for (int i = 0; i < nside; ++i)
+            SWIFT_HIP_TRY(hipStreamWaitEvent(w.side[i], w.fork, 0));
+        int k = 0;
+        for (int li = 0; li < nlive; ++li) {
+            {
+                const int seg = live[li], cls = seg / kRBins, R = kRBins - seg % kRBins;
                 BatchArgs b = ba;
                 b.task_lo = t.seg_task0 + seg;
                 b.task_hi = t.seg_task0 + seg + 1;
                 b.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
                 b.kc = cls == 0 ? kc16 : kc32;
-                hipStream_t s2 = w.side[k++ % Workspace::kSide];
+                hipStream_t s2 = w.side[k++ % nside];
                 SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, tuning().waves_per_cu, s2));
             }
         }
-        for (int i = 0; i < Workspace::kSide; ++i) {
+        for (int i = 0; i < nside; ++i) {
             SWIFT_HIP_TRY(hipEventRecord(w.join[i], w.side[i]));
             SWIFT_HIP_TRY(hipStreamWaitEvent(st, w.join[i], 0));
         }
         return kOk;
     };
-    SWIFT_HIP_TRY(mark(1, st));
-    if ((rc = launch_classes(t1, !force32, 0))) return rc;
+    SWIFT_HIP_TRY(mark(1, st, w.timed));
+    if ((rc = launch_classes(t1, !force32, 0, sync_sizes ? seg1 : nullptr))) return rc;
 
-    SWIFT_HIP_TRY(mark(2, st));
+    SWIFT_HIP_TRY(mark(2, st, w.timed));
     // ---- pass 2: int32 re-run of pairs flagged by the int16 kernel
     int64_t words2 = words1;
     if (!force32) {
@@ -345,7 +434,9 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
         if (sync_sizes) {
             SWIFT_HIP_TRY(hipMemcpyAsync(&words2, tot + 1, 8, hipMemcpyDeviceToHost, st));
             SWIFT_HIP_TRY(hipMemcpyAsync(&nre, cnt + 5, 4, hipMemcpyDeviceToHost, st));
+            SWIFT_HIP_TRY(hipMemcpyAsync(seg2, t2.seg_task0, sizeof(seg2), hipMemcpyDeviceToHost, st));
             SWIFT_HIP_TRY(hipStreamSynchronize(st));
+            w.stats.ntasks32 += seg2[2 * kRBins] - seg2[kRBins];
             w.stats.nrerun = nre;
             if ((size_t)words2 * 4 + 4 > w.trace.bytes) {
                 // grow, keeping pass-1 traces
@@ -363,30 +454,19 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
         }
         if (!sync_sizes || nre > 0) {
             ba.order = w.order2.as<int32_t>();
-            if ((rc = launch_classes(t2, false, 1))) return rc;
+            if ((rc = launch_classes(t2, false, 1, sync_sizes ? seg2 : nullptr))) return rc;
         }
     }
     w.stats.trace_words = words2;
 
     // ---- the walk-back ran inside the forward kernels (fused)
-    SWIFT_HIP_TRY(mark(3, st));
-    SWIFT_HIP_TRY(mark(4, st));
+    SWIFT_HIP_TRY(mark(3, st, w.timed));
+    SWIFT_HIP_TRY(mark(4, st, w.timed));
     if (sync_sizes) {
-        int32_t c[16];
-        SWIFT_HIP_TRY(hipMemcpyAsync(c, cnt, 16 * 4, hipMemcpyDeviceToHost, st));
         SWIFT_HIP_TRY(hipStreamSynchronize(st));
-        int32_t segbuf[2 * kRBins + 1];
-        SWIFT_HIP_TRY(hipMemcpy(segbuf, t1.seg_task0, sizeof(segbuf), hipMemcpyDeviceToHost));
-        w.stats.ntasks16 = force32 ? 0 : segbuf[kRBins];
-        w.stats.ntasks32 = segbuf[2 * kRBins] - segbuf[kRBins];
-        if (g_timing.on) {
+        if (g_timing.on && w.timed) {
             for (int i = 0; i < 4; ++i)
                 SWIFT_HIP_TRY(hipEventElapsedTime(&g_timing.ms[i], g_timing.ev[i], g_timing.ev[i + 1]));
-        }
-        if (c[kCntError]) {
-            set_error("pair lengths beyond maxq/maxr or the batched limits (%d/%d)", kMaxBatchQ,
-                      kMaxBatchR);
-            return kErrCapacity;
         }
     }
     return kOk;

@@ -52,6 +52,17 @@ int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32);
 int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync_sizes);
 int reserve_batch(int npairs, int maxq, int maxr);
 
+// Private batched-SW workspaces (device buffers, side streams), so that
+// several batches can be in flight at once, each driven by its own host
+// thread on its own stream.  sw_batch_run_in sizes the trace on the device
+// (it synchronises `st`).
+struct Workspace;
+Workspace *workspace_new();
+void workspace_free(Workspace *w);
+int64_t workspace_bytes(const Workspace *w);
+int sw_batch_run_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st);
+int64_t batch_bytes_estimate(int npairs, int maxq, int maxr);
+
 // statistics of the last batched run (for tests and the bench)
 struct BatchStats {
     int ntasks16, ntasks32, nrerun;

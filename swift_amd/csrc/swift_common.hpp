@@ -22,6 +22,7 @@ enum Status : int {
     kErrNoIndex = 3,   // phase-1 call before an index was built
     kErrIO = 4,        // file could not be read or written
     kErrCapacity = 5,  // input beyond a documented size limit
+    kErrTimeout = 6,   // a wait ended before the work finished
 };
 
 const char *last_error();

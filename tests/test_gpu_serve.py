@@ -1,0 +1,80 @@
+"""Serving loop on the GPU: requests batched together (continuous batching
+across concurrent slots) get exactly the records they get alone."""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def world():
+    from swift_amd import search, synth
+    rng = np.random.default_rng(55)
+    seqs = [synth.random_bases(rng, n, 0.41) for n in (300_000, 180_000, 90_000)]
+    reqs = []
+    for _ in range(36):
+        n = int(np.exp(rng.uniform(0, np.log(400))))
+        L = int(rng.integers(50, 301))
+        reads, _ = synth.sample_reads(rng, seqs, n, L, 0.015, 0.0015)
+        if rng.random() < 0.3:
+            reads.append(synth.random_bases(rng, L).tobytes())
+        reqs.append(reads)
+    ix = search.Index.from_sequences([s.tobytes() for s in seqs], 12, 1000)
+    return ix, reqs
+
+
+def same(a, b, ctx):
+    assert a.n == b.n, ctx
+    for k in b.cols:
+        assert np.array_equal(np.asarray(a.cols[k]), np.asarray(b.cols[k])), (ctx, k)
+    assert a.qry_cons == b.qry_cons and a.ref_cons == b.ref_cons, ctx
+
+
+def test_results_same_alone_and_in_mix(world):
+    from swift_amd.serve import Server
+    ix, reqs = world
+    alone = [ix.align_reads(r, 3) for r in reqs]
+    with Server(ix, nslots=4, mem_budget=16 << 30, max_batch_reads=4096) as srv:
+        ids = [None] * len(reqs)
+
+        def submit(lo):  # several client threads submit at once
+            for i in range(lo, len(reqs), 3):
+                ids[i] = srv.submit(reqs[i])
+
+        th = [threading.Thread(target=submit, args=(k,)) for k in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        got = [srv.result(i)[0] for i in ids]
+        st = srv.stats()
+    assert st["completed"] == len(reqs)
+    assert st["batches"] < len(reqs) and st["max_coalesced"] > 1  # requests were coalesced
+    for i, (g, a) in enumerate(zip(got, alone)):
+        same(g, a, i)
+
+
+def test_small_budget_many_batches(world):
+    from swift_amd.serve import Server
+    from swift_amd._lib import SwiftError
+    ix, reqs = world
+    small = [r for r in reqs if len(r) <= 64][:10]
+    with Server(ix, nslots=2, mem_budget=1 << 30, max_batch_reads=128) as srv:
+        ids = [srv.submit(r) for r in small]
+        got = [srv.result(i)[0] for i in ids]
+        with pytest.raises(SwiftError):
+            srv.submit([b"ACGT" * 25] * 129)   # larger than one slot admits
+    for g, r in zip(got, small):
+        same(g, ix.align_reads(r, 3), len(r))
+
+
+def test_one_slot_in_order(world):
+    from swift_amd.serve import Server
+    ix, reqs = world
+    with Server(ix, nslots=1, mem_budget=8 << 30, max_batch_reads=4096) as srv:
+        for r in reqs[:6]:
+            got, lat = srv.result(srv.submit(r))
+            assert lat > 0
+            same(got, ix.align_reads(r, 3), len(r))
