@@ -281,6 +281,8 @@ def main():
                     "index (for profiling)")
     ap.add_argument("--c2-only", action="store_true", help="only the C2 pipeline leg (no JSON "
                     "bench line contract; for profiling)")
+    ap.add_argument("--walk", choices=["default", "fused", "deferred"], default="default",
+                    help="traceback walk mode (default: the library's tuned setting)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -318,6 +320,9 @@ def main():
     from swift_amd import align, synth
     from swift_amd._lib import lib
     from swift_amd.device import synchronize
+    if args.walk != "default":
+        t = align.get_tuning()
+        align.set_tuning(t["waves_per_cu"], t["allow_int16"], args.walk == "deferred")
 
     t0 = time.perf_counter()
     qbuf, qoff, qlen, rbuf, roff, rlen = synth.c3_pairs_fast(args.pairs)
@@ -359,7 +364,7 @@ def main():
                                "U{36..250} bp, region = query + U{20..200} bp, 2% divergence; "
                                "scores, coordinates and consensus strings for every pair",
                    "pairs": args.pairs, "cells": cells, "scoring": [2, -1, -10, -1],
-                   "lanes": "packed int16 with int32 re-run"},
+                   "lanes": "packed int16 with int32 re-run", "tuning": align.get_tuning()},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
                      "traffic": round(TRAFFIC_B_PER_CELL * cells) if args.pairs == 1_000_000

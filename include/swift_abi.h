@@ -111,9 +111,12 @@ int swift_sw_batch_stats(int *ntasks16, int *ntasks32, int *nrerun, int64_t *tra
  * re-run pass and walk-back times in ms. */
 int swift_sw_batch_timing(int enable, float *ms4);
 
-/* Tuning knobs (set by the autotuner): persistent waves per CU and whether
- * the int16 path may be used. */
-int swift_set_tuning(int waves_per_cu, int allow_int16);
+/* Tuning knobs (set by the autotuner): persistent waves per CU (0 = the
+ * occupancy limit), whether the packed int16 path may be used, and whether
+ * the traceback walk runs fused into each wave task (0) or as one kernel
+ * after the sweeps (1).  Results are identical in every setting. */
+int swift_set_tuning(int waves_per_cu, int allow_int16, int walk_deferred);
+int swift_get_tuning(int *waves_per_cu, int *allow_int16, int *walk_deferred);
 
 /* ======================================================== phase 1: seeding
  * Reference drop-ins.  Same names, arguments and meaning as the reference's

@@ -158,5 +158,12 @@ def batch_stats():
     return dict(ntasks16=a.value, ntasks32=b.value, nrerun=c.value, trace_words=w.value)
 
 
-def set_tuning(waves_per_cu=8, allow_int16=True):
-    check(lib().swift_set_tuning(int(waves_per_cu), 1 if allow_int16 else 0))
+def set_tuning(waves_per_cu=0, allow_int16=True, walk_deferred=False):
+    check(lib().swift_set_tuning(int(waves_per_cu), 1 if allow_int16 else 0,
+                                 1 if walk_deferred else 0))
+
+
+def get_tuning() -> dict:
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    check(lib().swift_get_tuning(C.byref(a), C.byref(b), C.byref(c)))
+    return dict(waves_per_cu=a.value, allow_int16=bool(b.value), walk_deferred=bool(c.value))

@@ -105,13 +105,21 @@ int swift_sw_batch_timing(int enable, float *ms4) {
     return kOk;
 }
 
-int swift_set_tuning(int waves_per_cu, int allow_int16) {
+int swift_set_tuning(int waves_per_cu, int allow_int16, int walk_deferred) {
     if (waves_per_cu < 0 || waves_per_cu > 64) {
         set_error("waves_per_cu out of range");
         return kErrInvalid;
     }
     tuning().waves_per_cu = waves_per_cu;
     tuning().int16 = allow_int16 ? 1 : 0;
+    tuning().walk_deferred = walk_deferred ? 1 : 0;
+    return kOk;
+}
+
+int swift_get_tuning(int *waves_per_cu, int *allow_int16, int *walk_deferred) {
+    if (waves_per_cu) *waves_per_cu = tuning().waves_per_cu;
+    if (allow_int16) *allow_int16 = tuning().int16;
+    if (walk_deferred) *walk_deferred = tuning().walk_deferred;
     return kOk;
 }
 

@@ -69,7 +69,7 @@ struct Workspace {
     // plain path
     DevBuf p_trace, p_res, p_ptr;
     // batched path
-    DevBuf trace, res, ptr, bin_of, order, order2, tasks, task_words, scan_sums;
+    DevBuf trace, res, ptr, bin_of, order, order2, tasks, task_words, scan_sums, moves;
     DevBuf bins;      // bin_count, bin_cursor (x2 passes)
     DevBuf segs;      // seg_task0 / seg_first / seg_len (x2 passes)
     DevBuf counters;  // int32 counters
@@ -84,9 +84,9 @@ struct Workspace {
     bool timed = false;       // records the HIP-event phase marks (the shared workspace)
     std::mutex mu;            // one batched run at a time per workspace
 
-    DevBuf *all_bufs[17] = {&p_trace, &p_res, &p_ptr, &trace, &res, &ptr, &bin_of, &order,
-                            &order2, &tasks, &task_words, &scan_sums, &bins, &segs, &counters,
-                            &totals, nullptr};
+    DevBuf *all_bufs[18] = {&p_trace, &p_res, &p_ptr, &trace, &res, &ptr, &bin_of, &order,
+                            &order2, &tasks, &task_words, &scan_sums, &moves, &bins, &segs,
+                            &counters, &totals, nullptr};
 };
 
 static void init_workspace(Workspace *w) {
@@ -288,6 +288,9 @@ int reserve_batch(int npairs, int maxq, int maxr) {
 
 // counters layout (int32): [4*pass] ntasks, [4*pass+1] binned pairs, then
 // one work-queue cursor per (pass, segment) from kCntQueues on
+// 2-bit walk moves per pair id: room for the longest path any binned pair can
+// have (the kernel limits, not the caller's maxq/maxr, bound what is binned)
+constexpr int kMoveWords = (kMaxBatchQ + kMaxBatchR) / 16 + 1;
 constexpr int kCntError = 8;
 constexpr int kCntQueues = 16;
 constexpr int kQueuesPerPass = 2 * kRBins;
@@ -371,11 +374,13 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
         w.stats.ntasks16 = force32 ? 0 : seg1[kRBins];
         w.stats.ntasks32 = seg1[2 * kRBins] - seg1[kRBins];
     }
+    if (tuning().walk_deferred && (rc = w.moves.ensure((size_t)n * kMoveWords * 4))) return rc;
     SWIFT_HIP_TRY(launch_fill(cnt + kCntQueues, 2 * kQueuesPerPass, 0, st));
     BatchArgs ba{io.qseq, io.qoff, io.qlen, io.rseq, io.roff, io.rlen, w.order.as<int32_t>(),
                  w.tasks.as<Task>(), nullptr, nullptr, nullptr, w.trace.as<uint32_t>(),
                  w.res.as<PairResult>(), w.ptr.as<PairTrace>(), kc16, h_limit,
-                 io.score, io.start, io.end, io.alen, io.qcons, io.rcons, io.cons_off};
+                 io.score, io.start, io.end, io.alen, io.qcons, io.rcons, io.cons_off,
+                 tuning().walk_deferred};
     // One persistent kernel per (class, R) segment, each reading its task
     // range from the device (empty segments exit at once), spread longest
     // rows first over the side streams so the classes run concurrently and
@@ -409,6 +414,14 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                 b.kc = cls == 0 ? kc16 : kc32;
                 hipStream_t s2 = w.side[k++ % nside];
                 SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, tuning().waves_per_cu, s2));
+                if (ba.walk_deferred && seg_host) {
+                    // the class's walk follows its sweep on the same stream and
+                    // overlaps the sweeps of the classes still running
+                    const int slots = cls == 0 ? kSlots16 : kSlots32;
+                    const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;
+                    SWIFT_HIP_TRY(launch_walk(b, b.order, t.seg_first + seg, t.seg_len + seg, maxp,
+                                              w.moves.as<uint32_t>(), kMoveWords, s2));
+                }
             }
         }
         for (int i = 0; i < nside; ++i) {
@@ -459,8 +472,15 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     }
     w.stats.trace_words = words2;
 
-    // ---- the walk-back ran inside the forward kernels (fused)
+    // ---- walk-back: fused into the forward kernels; or, deferred, after each
+    // class's sweep (host-known class sizes) or else here, over every binned
+    // pair of pass 1 (re-run pairs carry their pass-2 results by now)
     SWIFT_HIP_TRY(mark(3, st, w.timed));
+    if (ba.walk_deferred && !sync_sizes) {
+        ba.order = w.order.as<int32_t>();
+        SWIFT_HIP_TRY(launch_walk(ba, w.order.as<int32_t>(), nullptr, cnt + 1, n,
+                                  w.moves.as<uint32_t>(), kMoveWords, st));
+    }
     SWIFT_HIP_TRY(mark(4, st, w.timed));
     if (sync_sizes) {
         SWIFT_HIP_TRY(hipStreamSynchronize(st));

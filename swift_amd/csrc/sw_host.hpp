@@ -79,6 +79,8 @@ const float *batch_timing_ms();
 struct Tuning {
     int waves_per_cu = 0;       // cap on persistent waves per CU (0: occupancy limit)
     int int16 = 1;              // allow the int16 packed path
+    int walk_deferred = 1;      // 1: walk-back kernels after each class's sweep (moves,
+                                // then strings); 0: the walk fused into each wave task
 };
 Tuning &tuning();
 

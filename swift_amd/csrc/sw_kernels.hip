@@ -290,12 +290,15 @@ struct ConsWriter {
 struct TraceReader {
     const uint4 *tr;
     PairTrace pt;
+    uint32_t inv;  // ceil(2^16 / R): x / R == (x * inv) >> 16 for R <= 20, x < 2048
     int64_t cached = -1;
     uint4 gran;
+    __device__ TraceReader(const uint4 *t, const PairTrace &p)
+        : tr(t), pt(p), inv((65536u + (uint32_t)p.rows - 1) / (uint32_t)p.rows) {}
     __device__ __forceinline__ int code(int i, int j) {
         const int R = pt.rows;
         const int kpw = (pt.half & 2) ? 8 : 16;
-        const int g = (i - 1) / R, k = (i - 1) - g * R;
+        const int g = (int)(((uint32_t)(i - 1) * inv) >> 16), k = (i - 1) - g * R;
         const int lane = pt.lane0 + g, t = (j - 1) + g;
         const int w = k / kpw;
         const int bit = (pt.half & 1) * 16 + 2 * (k - w * kpw);
@@ -332,16 +335,17 @@ struct WalkOut {
 // backtrack_cpu (src/smithWaterman.cu:263-335) over the packed direction
 // codes: emit the current cell while its pointer leads elsewhere; a pointer
 // into row 0 or column 0 emits the current cell and stops; a start pointer
-// stops without emitting.  Strings are written in walk order (the reference
-// stores them reversed) and NUL terminated.
-__device__ WalkOut walk_pair(const uint32_t *trace, int score, int row, int col,
-                             const PairTrace &pt, const char *qs, const char *rs, int qlen,
-                             int rlen, char *qcons, char *rcons) {
-    ConsWriter qc(qcons), rc(rcons);
+// stops without emitting.  Each emitted cell is handed to `emit(k, op, i, j)`
+// in walk order (the reference stores the strings reversed, i.e. in this
+// order) with op 0 = diagonal (both bases), 1 = up (reference gap), 2 = left
+// (query gap).
+template <class Emit>
+__device__ __forceinline__ WalkOut walk_cells(const uint32_t *trace, int score, int row, int col,
+                                              const PairTrace &pt, int qlen, int rlen, Emit &emit) {
     int start = 0, nrb = -1, tick = 0;
     if (score > 0 && row >= 1 && row <= qlen && col >= 1 && col <= rlen) {
         int ci = row, cj = col, ni, nj;
-        TraceReader rd{reinterpret_cast<const uint4 *>(trace), pt};
+        TraceReader rd(reinterpret_cast<const uint4 *>(trace), pt);
         auto back = [&](int i, int j, int &oi, int &oj) {
             int c = rd.code(i, j);
             oi = (c & 2) ? i - 1 : i;            // diag (3) and up (2) move up
@@ -350,9 +354,9 @@ __device__ WalkOut walk_pair(const uint32_t *trace, int score, int row, int col,
         back(ci, cj, ni, nj);
         for (;;) {
             if (ci == ni && cj == nj) break;
-            if (nj == cj) rc.put('-');
-            else { rc.put(rs[cj - 1]); ++nrb; }
-            qc.put((ni == ci) ? '-' : qs[ci - 1]);
+            const int op = nj == cj ? 1 : ni == ci ? 2 : 0;
+            if (op != 1) ++nrb;
+            emit(tick, op, ci, cj);
             start = cj;
             ++tick;
             if (ni == 0 || nj == 0) break;  // the target is in row/column 0
@@ -361,9 +365,22 @@ __device__ WalkOut walk_pair(const uint32_t *trace, int score, int row, int col,
             back(ci, cj, ni, nj);
         }
     }
+    return WalkOut{start, start + nrb, tick};
+}
+
+// Walk emitting the consensus strings directly (fused mode), NUL terminated.
+__device__ WalkOut walk_pair(const uint32_t *trace, int score, int row, int col,
+                             const PairTrace &pt, const char *qs, const char *rs, int qlen,
+                             int rlen, char *qcons, char *rcons) {
+    ConsWriter qc(qcons), rc(rcons);
+    auto emit = [&](int, int op, int i, int j) {
+        rc.put(op == 1 ? '-' : rs[j - 1]);
+        qc.put(op == 2 ? '-' : qs[i - 1]);
+    };
+    WalkOut wo = walk_cells(trace, score, row, col, pt, qlen, rlen, emit);
     qc.finish();
     rc.finish();
-    return WalkOut{start, start + nrb, tick};
+    return wo;
 }
 
 // Makes this wave's earlier global stores (trace words) visible to its own
@@ -578,12 +595,20 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
             a.res[pid[h]] = PairResult{gres[h].score, gres[h].row, gres[h].col, gres[h].flag};
     }
 
+    const int s = lane;
+    const int sg = min(s / PPG, 3), sh = s % PPG;
+    if (a.walk_deferred) {
+        // record where the pair's trace lives; sw_walk_kernel walks it
+        if (s < tk.npairs)
+            a.ptr[a.order[tk.first + s]] = PairTrace{tk.trace_base, sg * kBatchG, (int16_t)R,
+                                                     (int8_t)S::W, (int8_t)(sh | (PK ? 2 : 0))};
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
     // Traceback of the task's pairs right after the sweep, while the trace
     // is cache-hot: lane s walks slot s.  A flagged pair is left to the int32
     // re-run.  The walk is latency-bound; other waves' sweeps cover it.
     wave_store_fence();
-    const int s = lane;
-    const int sg = min(s / PPG, 3), sh = s % PPG;
     int w_score = 0, w_row = 0, w_col = 0, w_flag = 1;
 #pragma unroll
     for (int h = 0; h < PPG; ++h) {
@@ -612,6 +637,146 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
         a.alen_out[p] = wo.len;
     }
     __builtin_amdgcn_wave_barrier();
+}
+
+// Orders one wave's LDS stores before its later LDS loads from other lanes
+// (each wave of sw_expand_kernel works alone on its own LDS region, so no
+// workgroup barrier is involved).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Deferred walk-back, in two kernels over the binned pairs after every sweep
+// of the call (both passes) has finished.  sw_walk_kernel: one lane per pair
+// follows the trace and stores the path as 2-bit moves (16 per word, walk
+// order) plus start/end/length; it touches no sequence bytes, so its
+// dependent loads are the trace granules alone.  sw_expand_kernel: one wave
+// per pair turns the moves into the consensus strings: lane l takes moves
+// [16l, 16l+16), a wave prefix sum of the row/column steps gives each lane
+// its starting cell, and the lanes read and write their stretch of the
+// sequences side by side.  A pair still flagged for the int32 re-run (none
+// after the re-run pass) is skipped.
+__global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t *order,
+                                                      const int32_t *first_p,
+                                                      const int32_t *count, uint32_t *moves,
+                                                      int move_words) {
+    const int n = *count;
+    const int32_t *ord = order + (first_p ? *first_p : 0);
+    for (int i = blockIdx.x * 256 + (int)threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int p = ord[i];
+        const PairResult r = a.res[p];
+        if (r.flags & 1) continue;
+        const PairTrace pt = a.ptr[p];
+        uint32_t *mv = moves + (int64_t)p * move_words;
+        uint32_t word = 0;
+        auto emit = [&](int k, int op, int, int) {
+            word |= (uint32_t)op << (2 * (k & 15));
+            if ((k & 15) == 15) {
+                mv[k >> 4] = word;
+                word = 0;
+            }
+        };
+        WalkOut wo = walk_cells(a.trace, r.score, r.row, r.col, pt, a.qlen[p], a.rlen[p], emit);
+        if (wo.len & 15) mv[wo.len >> 4] = word;
+        a.score_out[p] = r.score;
+        a.start_out[p] = wo.start;
+        a.end_out[p] = wo.end;
+        a.alen_out[p] = wo.len;
+    }
+}
+
+__global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32_t *order,
+                                                        const int32_t *first_p,
+                                                        const int32_t *count,
+                                                        const uint32_t *moves, int move_words) {
+    // per wave: the pair's query and reference bytes, then both output strings
+    constexpr int kOut = kMaxBatchQ + kMaxBatchR + 4;
+    __shared__ unsigned char lq[4][kMaxBatchQ], lr[4][kMaxBatchR];
+    __shared__ unsigned char oq[4][kOut], orr[4][kOut];
+    const int n = *count;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t *ord = order + (first_p ? *first_p : 0);
+    for (int i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
+        const int p = ord[i];
+        const PairResult r = a.res[p];
+        if (r.flags & 1) continue;
+        const int len = a.alen_out[p];
+        const int64_t co = a.coff[p];
+        char *qc = a.qcons + co, *rc = a.rcons + co;
+        const int ql = a.qlen[p], rl = a.rlen[p];
+        const unsigned char *qs = (const unsigned char *)a.qseq + a.qoff[p];
+        const unsigned char *rs = (const unsigned char *)a.rseq + a.roff[p];
+        // the path only visits rows <= row and columns <= col
+        const int nq = min(ql, r.row), nr = min(rl, r.col);
+        for (int k = lane; k < nq; k += 64) lq[wv][k] = qs[k];
+        for (int k = lane; k < nr; k += 64) lr[wv][k] = rs[k];
+        const int nw = (len + 15) >> 4;
+        const uint32_t w = lane < nw ? moves[(int64_t)p * move_words + lane] : 0u;
+        const int cnt = lane < nw ? min(16, len - 16 * lane) : 0;
+        // row steps: diagonal and up moves; column steps: diagonal and left
+        int di = 0, dj = 0;
+        for (int k = 0; k < cnt; ++k) {
+            const int op = (w >> (2 * k)) & 3;
+            di += op != 2;
+            dj += op != 1;
+        }
+        int ei = di, ej = dj;  // inclusive wave scans
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int oi = __shfl_up(ei, d, 64), oj = __shfl_up(ej, d, 64);
+            if (lane >= d) {
+                ei += oi;
+                ej += oj;
+            }
+        }
+        wave_lds_sync();  // this wave's lq/lr filled
+        int ci = r.row - (ei - di), cj = r.col - (ej - dj);
+        for (int k = 0; k < cnt; ++k) {
+            const int op = (w >> (2 * k)) & 3;
+            const int at = 16 * lane + k;
+            oq[wv][at] = op == 2 ? '-' : lq[wv][ci - 1];
+            orr[wv][at] = op == 1 ? '-' : lr[wv][cj - 1];
+            ci -= op != 2;
+            cj -= op != 1;
+        }
+        if (lane == 0) {
+            oq[wv][len] = 0;
+            orr[wv][len] = 0;
+        }
+        wave_lds_sync();
+        // coalesced copy out (NUL included); dword stores when aligned
+        if ((co & 3) == 0) {
+            const int nd = (len + 1 + 3) >> 2;
+            for (int k = lane; k < nd; k += 64) {
+                uint32_t vq = 0, vr = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int at = 4 * k + b;
+                    if (at <= len) {
+                        vq |= (uint32_t)oq[wv][at] << (8 * b);
+                        vr |= (uint32_t)orr[wv][at] << (8 * b);
+                    }
+                }
+                if (4 * k + 3 <= len) {
+                    reinterpret_cast<uint32_t *>(qc)[k] = vq;
+                    reinterpret_cast<uint32_t *>(rc)[k] = vr;
+                } else {
+                    for (int b = 0; 4 * k + b <= len; ++b) {
+                        qc[4 * k + b] = (char)(vq >> (8 * b));
+                        rc[4 * k + b] = (char)(vr >> (8 * b));
+                    }
+                }
+            }
+        } else {
+            for (int k = lane; k <= len; k += 64) {
+                qc[k] = (char)oq[wv][k];
+                rc[k] = (char)orr[wv][k];
+            }
+        }
+        wave_lds_sync();  // before the next pair overwrites this wave's buffers
+    }
 }
 
 // Row classes: R per lane with 16-lane groups covers query length 16*R.
@@ -921,6 +1086,17 @@ hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count,
         default: return hipErrorInvalidValue;
         }
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t *first,
+                       const int32_t *count, int max_pairs, uint32_t *moves, int move_words,
+                       hipStream_t st) {
+    if (max_pairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sw_walk_kernel, dim3((max_pairs + 255) / 256), dim3(256), 0, st, ba, order,
+                       first, count, moves, move_words);
+    hipLaunchKernelGGL(sw_expand_kernel, dim3((max_pairs + 3) / 4), dim3(256), 0, st, ba, order,
+                       first, count, moves, move_words);
     return hipGetLastError();
 }
 

@@ -79,6 +79,7 @@ struct BatchArgs {
     int32_t *score_out, *start_out, *end_out, *alen_out;
     char *qcons, *rcons;
     const int64_t *coff;
+    int walk_deferred;        // 1: record PairTrace descriptors, walk in sw_walk_kernel
 };
 
 // outputs of the plain (alignSequences) kernel: reference layout
@@ -119,6 +120,12 @@ struct TaskArgs {
 // launch helpers (sw_kernels.hip)
 hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
                         hipStream_t st);
+// deferred walk-back over pairs order[*first .. *first + *count) (first may
+// be null: 0; max_pairs bounds *count); moves holds move_words words per pair
+// id, move_words >= (kMaxBatchQ + kMaxBatchR) / 16 + 1
+hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t *first,
+                       const int32_t *count, int max_pairs, uint32_t *moves, int move_words,
+                       hipStream_t st);
 // scan_sums: room for max_tasks / 2048 + 1 int64 values
 hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order, Task *tasks,
                           int64_t *task_words, const int64_t *base0, int64_t *total_words,

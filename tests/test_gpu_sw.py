@@ -177,3 +177,31 @@ def test_batch_repeatable(A):
     b.run()
     r2 = b.results()
     assert np.array_equal(r1.score, r2.score) and r1.qry_cons == r2.qry_cons
+
+
+@pytest.mark.parametrize("deferred", [False, True])
+def test_batch_walk_modes(A, orc, deferred):
+    """Fused and deferred walk-back give identical results (C3 shapes, an
+    int16-overflow re-run and exotic bytes in the same call)."""
+    from swift_amd import synth
+    before = A.get_tuning()
+    try:
+        A.set_tuning(before["waves_per_cu"], True, deferred)
+        qs, rs = synth.c3_pairs(1500, seed=17)
+        rnd = random.Random(17)
+        q2, r2 = rand_pairs(rnd, 200, 120, 300, [b"ACGTRY*", b"ACGTacgtNn#"])
+        qs, rs = qs + q2, rs + r2
+        res = A.sw_batch(qs, rs)
+        sc, st, en, al, qc, rc = orc.sw_batch(qs, rs, nthreads=8)
+        assert np.array_equal(res.score, sc) and np.array_equal(res.start, st)
+        assert np.array_equal(res.end, en) and np.array_equal(res.length, al)
+        assert res.qry_cons == qc and res.ref_cons == rc
+        big = A.Scoring(200, -1, -10, -1)
+        reg = [bytes(rnd.choice(b"ACGT") for _ in range(rnd.randint(60, 300))) for _ in range(50)]
+        qo = [g[5:5 + rnd.randint(20, 50)] for g in reg]
+        res = A.sw_batch(qo, reg, big)
+        assert A.batch_stats()["nrerun"] > 0
+        for i in range(len(qo)):
+            compare(res.record(i), orc.sw(reg[i], qo[i], 200, -1, -10, -1), i)
+    finally:
+        A.set_tuning(before["waves_per_cu"], before["allow_int16"], before["walk_deferred"])
