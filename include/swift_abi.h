@@ -235,6 +235,22 @@ int swift_align_reads(const swift_index *ix, const char *reads_d, const int64_t 
                       int match, int mismatch, int gap_open, int gap_ext,
                       const swift_records *rec, void *stream);
 
+/* Phase 1 -> phase 2 captured as one hipGraph for a fixed batch shape: the
+ * read buffers (nreads slots; an unused slot holds a length-0 read) and the
+ * record buffers are fixed device addresses; the caller rewrites their
+ * contents between launches.  Scratch is reserved for the worst case when
+ * the graph is made.  After a launch and a synchronisation of its stream,
+ * swift_pipeline_graph_check reports length and capacity errors of that
+ * run.  Records equal swift_align_reads on the same reads. */
+typedef struct swift_graph swift_graph;
+int swift_pipeline_graph_create(const swift_index *ix, const char *reads_d, const int64_t *off_d,
+                                const int32_t *len_d, int nreads, int max_read_len, int max_hits,
+                                int match, int mismatch, int gap_open, int gap_ext,
+                                const swift_records *rec, swift_graph **out);
+int swift_pipeline_graph_launch(swift_graph *g, void *stream);
+int swift_pipeline_graph_check(swift_graph *g);
+int swift_pipeline_graph_free(swift_graph *g);
+
 /* ======================================================== serving loop
  * No reference counterpart (the reference is a batch command-line tool).
  * A server keeps an index resident and aligns read batches submitted from

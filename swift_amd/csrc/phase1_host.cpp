@@ -367,9 +367,10 @@ static int align_reads(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     return align_reads_in(map_work(), nullptr, ix, rd, max_read_len, max_hits, sc, rec, st);
 }
 
-int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDev &rd,
-                   int max_read_len, int max_hits, const Scoring &sc, const swift_records &rec,
-                   hipStream_t st) {
+// Validates a pipeline call and sizes its scratch; fills the launch
+// arguments of the mapping (m) and of the records (p).
+int align_prepare(MapWork &w, const IndexObj &ix, const ReadsDev &rd, int max_read_len,
+                  int max_hits, const swift_records &rec, MapArgs *m, PairsDev *p) {
     if (max_read_len > kMaxBatchQ) {
         set_error("the pipeline aligns reads up to %d bases (got %d)", kMaxBatchQ, max_read_len);
         return kErrCapacity;
@@ -384,8 +385,7 @@ int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDe
         set_error("cons_stride %lld below max_read_len + 302", (long long)rec.cons_stride);
         return kErrInvalid;
     }
-    MapArgs m;
-    int rc = prepare_map(ix, rd, max_read_len, max_hits, &m, w, true);
+    int rc = prepare_map(ix, rd, max_read_len, max_hits, m, w, true);
     if (rc) return rc;
     const size_t c = (size_t)std::max<int64_t>(cap, 1);
     if ((rc = ensure(w.pair0, 4 * (2 * (size_t)rd.nreads + 16)))) return rc;
@@ -395,25 +395,33 @@ int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDe
     if ((rc = ensure(w.coff, 8 * c))) return rc;
     if ((rc = ensure(w.qlen, 4 * c))) return rc;
     if ((rc = ensure(w.rlen, 4 * c))) return rc;
+    p->npairs = rec.nrecords;
+    p->pair0 = w.pair0.as<int32_t>();
+    p->read = rec.read;
+    p->strand = rec.strand;
+    p->ref = rec.ref_idx;
+    p->shift = rec.shift;
+    p->pos = rec.ref_pos;
+    p->nbig = rec.nbiggest;
+    p->rdist = rec.ref_distance;
+    p->qoff = w.qoff.as<int64_t>();
+    p->roff = w.roff.as<int64_t>();
+    p->coff = w.coff.as<int64_t>();
+    p->qlen = w.qlen.as<int32_t>();
+    p->rlen = w.rlen.as<int32_t>();
+    p->capacity = (int)cap;
+    p->cstride = rec.cons_stride;
+    p->scan_sums = w.scan_sums.as<int32_t>();
+    return kOk;
+}
+
+// Enqueues map -> records/windows -> batched SW -> best choice on `st`.
+// sync: the batched SW sizes its trace on the device (synchronising);
+// otherwise it runs from a reservation and the sequence is capturable.
+int align_enqueue(const MapArgs &m, const PairsDev &p, Workspace *sws, int nreads, int max_read_len,
+                  const Scoring &sc, const swift_records &rec, hipStream_t st, bool sync) {
+    int rc;
     if ((rc = run_map(m, st))) return rc;
-    PairsDev p;
-    p.npairs = rec.nrecords;
-    p.pair0 = w.pair0.as<int32_t>();
-    p.read = rec.read;
-    p.strand = rec.strand;
-    p.ref = rec.ref_idx;
-    p.shift = rec.shift;
-    p.pos = rec.ref_pos;
-    p.nbig = rec.nbiggest;
-    p.rdist = rec.ref_distance;
-    p.qoff = w.qoff.as<int64_t>();
-    p.roff = w.roff.as<int64_t>();
-    p.coff = w.coff.as<int64_t>();
-    p.qlen = w.qlen.as<int32_t>();
-    p.rlen = w.rlen.as<int32_t>();
-    p.capacity = (int)cap;
-    p.cstride = rec.cons_stride;
-    p.scan_sums = w.scan_sums.as<int32_t>();
     SWIFT_HIP_TRY(launch_records(m, p, kBefore, kWin, st));
     BatchIO io;
     io.qseq = m.strand_seq;
@@ -422,7 +430,7 @@ int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDe
     io.rseq = m.ix.seq;
     io.roff = p.roff;
     io.rlen = p.rlen;
-    io.npairs = (int)cap;
+    io.npairs = p.capacity;
     io.maxq = max_read_len;
     io.maxr = kWin;
     io.score = rec.score;
@@ -432,12 +440,32 @@ int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDe
     io.qcons = rec.qry_cons;
     io.rcons = rec.ref_cons;
     io.cons_off = p.coff;
-    if ((rc = sws ? sw_batch_run_in(sws, io, sc, st) : sw_batch_run(io, sc, st, true))) return rc;
-    SWIFT_HIP_TRY(launch_choose_best(p, rd.nreads, rec.score, rec.keep, rec.ntied, st));
+    if (sync)
+        rc = sws ? sw_batch_run_in(sws, io, sc, st) : sw_batch_run(io, sc, st, true);
+    else
+        rc = sw_batch_enqueue_in(sws, io, sc, st);
+    if (rc) return rc;
+    SWIFT_HIP_TRY(launch_choose_best(p, nreads, rec.score, rec.keep, rec.ntied, st));
+    return kOk;
+}
+
+// mapping error flags of the last run on `st` (synchronises)
+int map_check(const MapArgs &m, hipStream_t st) {
     int32_t flags[2];
     SWIFT_HIP_TRY(hipMemcpyAsync(flags, m.ovl_count, 8, hipMemcpyDeviceToHost, st));
     SWIFT_HIP_TRY(hipStreamSynchronize(st));
     return map_error(flags[1]);
+}
+
+int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDev &rd,
+                   int max_read_len, int max_hits, const Scoring &sc, const swift_records &rec,
+                   hipStream_t st) {
+    MapArgs m;
+    PairsDev p;
+    int rc = align_prepare(w, ix, rd, max_read_len, max_hits, rec, &m, &p);
+    if (rc) return rc;
+    if ((rc = align_enqueue(m, p, sws, rd.nreads, max_read_len, sc, rec, st, true))) return rc;
+    return map_check(m, st);
 }
 
 // ---------------------------------------------------------------------------

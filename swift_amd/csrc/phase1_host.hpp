@@ -75,6 +75,15 @@ int ensure(DevArr &a, size_t bytes);
 int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDev &rd,
                    int max_read_len, int max_hits, const Scoring &sc, const swift_records &rec,
                    hipStream_t st);
+// The same in steps: align_prepare sizes scratch and fills the launch
+// arguments; align_enqueue launches (sync = false: no host synchronisation
+// or allocation, sws required and reserved, capturable); map_check reads the
+// mapping error flags afterwards.
+int align_prepare(MapWork &w, const IndexObj &ix, const ReadsDev &rd, int max_read_len,
+                  int max_hits, const swift_records &rec, MapArgs *m, PairsDev *p);
+int align_enqueue(const MapArgs &m, const PairsDev &p, Workspace *sws, int nreads, int max_read_len,
+                  const Scoring &sc, const swift_records &rec, hipStream_t st, bool sync);
+int map_check(const MapArgs &m, hipStream_t st);
 // device bytes align_reads_in may hold for nreads reads of <= max_read_len
 int64_t align_bytes_estimate(int nreads, int max_read_len, int max_hits);
 

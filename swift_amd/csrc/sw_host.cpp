@@ -240,6 +240,9 @@ void set_plain_budget_words(size_t words) { g_plain_budget_words = std::max<size
 // ---------------------------------------------------------------------------
 // Batched path.
 // ---------------------------------------------------------------------------
+// 2-bit walk moves per pair id: room for the longest path any binned pair can
+// have (the kernel limits, not the caller's maxq/maxr, bound what is binned)
+constexpr int kMoveWords = (kMaxBatchQ + kMaxBatchR) / 16 + 1;
 static int max_tasks_for(int npairs) { return (npairs + kSlots32 - 1) / kSlots32 + 2 * kRBins + 1; }
 
 static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
@@ -278,19 +281,25 @@ int64_t batch_bytes_estimate(int npairs, int maxq, int maxr) {
            (int64_t)4 * kNumBins * 4 + (1 << 16);
 }
 
-int reserve_batch(int npairs, int maxq, int maxr) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    Workspace &w = ws();
+static int reserve_all(Workspace &w, int npairs, int maxq, int maxr) {
     int rc = reserve_common(w, npairs);
     if (rc) return rc;
+    if ((rc = w.moves.ensure((size_t)std::max(npairs, 1) * kMoveWords * 4))) return rc;
     return w.trace.ensure((size_t)worst_trace_words(npairs, maxq, maxr) * 4);
+}
+
+int reserve_batch(int npairs, int maxq, int maxr) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return reserve_all(ws(), npairs, maxq, maxr);
+}
+
+int reserve_batch_in(Workspace *w, int npairs, int maxq, int maxr) {
+    std::lock_guard<std::mutex> lk(w->mu);
+    return reserve_all(*w, npairs, maxq, maxr);
 }
 
 // counters layout (int32): [4*pass] ntasks, [4*pass+1] binned pairs, then
 // one work-queue cursor per (pass, segment) from kCntQueues on
-// 2-bit walk moves per pair id: room for the longest path any binned pair can
-// have (the kernel limits, not the caller's maxq/maxr, bound what is binned)
-constexpr int kMoveWords = (kMaxBatchQ + kMaxBatchR) / 16 + 1;
 constexpr int kCntError = 8;
 constexpr int kCntQueues = 16;
 constexpr int kQueuesPerPass = 2 * kRBins;
@@ -305,6 +314,22 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
 int sw_batch_run_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st) {
     std::lock_guard<std::mutex> lk(w->mu);
     return batch_run(*w, io, sc, st, true);
+}
+
+int sw_batch_enqueue_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(w->mu);
+    return batch_run(*w, io, sc, st, false);
+}
+
+int sw_batch_check_in(Workspace *w, hipStream_t st) {
+    int32_t c[16];
+    SWIFT_HIP_TRY(hipMemcpyAsync(c, w->counters.ptr, sizeof(c), hipMemcpyDeviceToHost, st));
+    SWIFT_HIP_TRY(hipStreamSynchronize(st));
+    if (c[kCntError]) {
+        set_error("pair lengths beyond the batched limits (%d/%d)", kMaxBatchQ, kMaxBatchR);
+        return kErrCapacity;
+    }
+    return kOk;
 }
 
 static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStream_t st,

@@ -61,6 +61,13 @@ Workspace *workspace_new();
 void workspace_free(Workspace *w);
 int64_t workspace_bytes(const Workspace *w);
 int sw_batch_run_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st);
+// Capturable form: no host synchronisation, no allocation (reserve_batch_in
+// first, for at least this npairs / maxq / maxr); every row class is launched
+// and empty ones exit at once.  sw_batch_check_in reads the length-error flag
+// afterwards (it synchronises `st`).
+int reserve_batch_in(Workspace *w, int npairs, int maxq, int maxr);
+int sw_batch_enqueue_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st);
+int sw_batch_check_in(Workspace *w, hipStream_t st);
 int64_t batch_bytes_estimate(int npairs, int maxq, int maxr);
 
 // statistics of the last batched run (for tests and the bench)

@@ -210,6 +210,64 @@ class Records:
         return d
 
 
+class PipelineGraph:
+    """Phase 1 -> phase 2 captured as one hipGraph over fixed device buffers:
+    `slots` read slots of up to `max_read_len` bases.  run(reads) rewrites
+    the read buffers (unused slots become length-0 reads), replays the graph
+    and returns the records."""
+
+    def __init__(self, index: Index, slots: int, max_read_len: int, max_hits=3,
+                 scoring=Scoring()):
+        self.index, self.slots, self.max_len = index, int(slots), int(max_read_len)
+        self.seq = DeviceBuffer(self.slots * self.max_len + 64)
+        off = np.arange(self.slots, dtype=np.int64) * self.max_len
+        self.off = DeviceBuffer.from_array(off)
+        self.len = DeviceBuffer(4 * self.slots)
+        self.len.upload(np.zeros(self.slots, np.int32))
+        self.out = RecordBuffers(2 * self.slots * max_hits, self.max_len)
+        self._h = C.c_void_p()
+        check(lib().swift_pipeline_graph_create(
+            index._h, self.seq.cptr, self.off.cptr, self.len.cptr, self.slots, self.max_len,
+            max_hits, scoring.match, scoring.mismatch, scoring.gap_open, scoring.gap_ext,
+            C.byref(self.out.c), C.byref(self._h)))
+
+    def load(self, reads) -> None:
+        if len(reads) > self.slots:
+            raise SwiftError(f"{len(reads)} reads for {self.slots} slots")
+        buf = np.zeros(self.slots * self.max_len, np.uint8)
+        lens = np.zeros(self.slots, np.int32)
+        for i, r in enumerate(reads):
+            r = bytes(r)
+            if len(r) > self.max_len:
+                raise SwiftError(f"read {i} longer than {self.max_len}")
+            buf[i * self.max_len:i * self.max_len + len(r)] = np.frombuffer(r, np.uint8)
+            lens[i] = len(r)
+        self.seq.upload(buf)
+        self.len.upload(lens)
+
+    def launch(self, stream=None) -> None:
+        check(lib().swift_pipeline_graph_launch(self._h, stream))
+
+    def run(self, reads, strings=True) -> "Records":
+        from .device import synchronize
+        self.load(reads)
+        self.launch()
+        synchronize()
+        check(lib().swift_pipeline_graph_check(self._h))
+        return self.out.results(strings)
+
+    def close(self):
+        if self._h:
+            lib().swift_pipeline_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- drop-ins
 class Query(C.Structure):
     """struct queryStruct (src/query.h:29-37)."""
@@ -270,6 +328,7 @@ def search_queries3(qry_path, ref_path, match_path, k=12, max_hits=3, thr=1000,
     return nm.value
 
 
-__all__ = ["Index", "ReadBatch", "StrandHits", "RecordBuffers", "Records", "Query", "get_hash",
+__all__ = ["Index", "ReadBatch", "StrandHits", "RecordBuffers", "Records", "PipelineGraph",
+           "Query", "get_hash",
            "revcomp", "lookup_table5_create", "lookup_table5_map_qry", "lookup_table5_delete",
            "map_hits6_best_hits", "search_queries3", "SwiftError"]
