@@ -281,6 +281,8 @@ def main():
                     "index (for profiling)")
     ap.add_argument("--c2-only", action="store_true", help="only the C2 pipeline leg (no JSON "
                     "bench line contract; for profiling)")
+    ap.add_argument("--autotune", type=int, default=200000, metavar="PAIRS",
+                    help="autotune the batched path on this many workload pairs first (0: off)")
     ap.add_argument("--walk", choices=["default", "fused", "deferred"], default="default",
                     help="traceback walk mode (default: the library's tuned setting)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -330,6 +332,15 @@ def main():
     log(f"[bench] generated {args.pairs} pairs, {cells:.3e} cells in "
         f"{time.perf_counter() - t0:.1f} s")
 
+    tuned = None
+    if args.autotune > 0 and args.walk == "default":
+        k = min(args.autotune, args.pairs)
+        sample = align.PairBatch.from_arrays(qbuf, qoff[:k], qlen[:k], rbuf, roff[:k], rlen[:k])
+        t0 = time.perf_counter()
+        tuned = sample.autotune()
+        log(f"[bench] autotune on {k} pairs ({time.perf_counter() - t0:.1f} s): "
+            f"{ {x: tuned[x] for x in ('waves_per_cu', 'allow_int16', 'walk_deferred', 'ms')} }")
+        del sample
     batch = align.PairBatch.from_arrays(qbuf, qoff, qlen, rbuf, roff, rlen)
     ms4 = (C.c_float * 4)()
     lib().swift_sw_batch_timing(1, ms4)
@@ -377,6 +388,12 @@ def main():
                       "walk": round(float(np.mean(walk)), 3)},
         "batch_stats": stats,
     }
+    if tuned is not None:
+        out["autotune"] = {"sample_pairs": min(args.autotune, args.pairs),
+                           "chosen": {x: tuned[x] for x in ("waves_per_cu", "allow_int16",
+                                                             "walk_deferred")},
+                           "candidates_ms": [[r["waves_per_cu"], r["int16"], r["walk_deferred"],
+                                              r["ms"]] for r in tuned["table"]]}
     if not args.no_plain:
         groups = plain_pass_setup(qbuf, qoff, qlen, rbuf, roff, rlen)
         plain_pass(groups)

@@ -118,6 +118,23 @@ int swift_sw_batch_timing(int enable, float *ms4);
 int swift_set_tuning(int waves_per_cu, int allow_int16, int walk_deferred);
 int swift_get_tuning(int *waves_per_cu, int *allow_int16, int *walk_deferred);
 
+/* Autotuner of the batched path: times every candidate setting (waves per
+ * CU in {max, 8, 12, 16, 20, 24} x walk fused/deferred with int16 lanes, and
+ * int32-only lanes) on the given pairs, reps timed runs each after a warm-up,
+ * with HIP events on `stream`, then applies the fastest.  Outputs are
+ * written by the timed runs (identical under every setting).  table_ms
+ * (optional, 4 floats per candidate: waves, int16, walk, ms) receives up to
+ * table_cap rows; *ntable the number of candidates timed.  Changes the
+ * process-wide tuning: call it while no other batched call is running. */
+int swift_autotune_batch(const char *qseq_d, const int64_t *qoff_d, const int32_t *qlen_d,
+                         const char *rseq_d, const int64_t *roff_d, const int32_t *rlen_d,
+                         int npairs, int maxq, int maxr, int match, int mismatch, int gap_open,
+                         int gap_ext, int32_t *score_d, int32_t *start_d, int32_t *end_d,
+                         int32_t *alen_d, char *qcons_d, char *rcons_d,
+                         const int64_t *cons_off_d, int reps, void *stream, int *best_waves,
+                         int *best_int16, int *best_walk, float *best_ms, float *table_ms,
+                         int table_cap, int *ntable);
+
 /* ======================================================== phase 1: seeding
  * Reference drop-ins.  Same names, arguments and meaning as the reference's
  * CPU phase 1; the index they share lives in device memory. */

@@ -145,6 +145,27 @@ class PairBatch:
         return Alignments(sc, st, en, al, qs, rs)
 
 
+    def autotune(self, scoring=Scoring(), reps=2) -> dict:
+        """Times the batched path's settings on these pairs and applies the
+        fastest (see swift_autotune_batch)."""
+        table = (C.c_float * (4 * 32))()
+        bw, bi, bk, nt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        bms = C.c_float()
+        check(lib().swift_autotune_batch(
+            C.c_void_p(self.q.ptr), C.c_void_p(self.qo.ptr), C.c_void_p(self.ql.ptr),
+            C.c_void_p(self.r.ptr), C.c_void_p(self.ro.ptr), C.c_void_p(self.rl.ptr),
+            self.n, self.maxq, self.maxr, scoring.match, scoring.mismatch, scoring.gap_open,
+            scoring.gap_ext, C.c_void_p(self.score.ptr), C.c_void_p(self.start.ptr),
+            C.c_void_p(self.end.ptr), C.c_void_p(self.alen.ptr), C.c_void_p(self.qc.ptr),
+            C.c_void_p(self.rc.ptr), C.c_void_p(self.co.ptr), reps, None, C.byref(bw),
+            C.byref(bi), C.byref(bk), C.byref(bms), table, 32, C.byref(nt)))
+        rows = [dict(waves_per_cu=int(table[4 * i]), int16=bool(table[4 * i + 1]),
+                     walk_deferred=bool(table[4 * i + 2]), ms=round(float(table[4 * i + 3]), 3))
+                for i in range(nt.value)]
+        return dict(waves_per_cu=bw.value, allow_int16=bool(bi.value),
+                    walk_deferred=bool(bk.value), ms=round(bms.value, 3), table=rows)
+
+
 def sw_batch(queries, refs, scoring=Scoring()) -> Alignments:
     b = PairBatch(queries, refs)
     b.run(scoring)

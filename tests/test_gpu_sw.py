@@ -205,3 +205,23 @@ def test_batch_walk_modes(A, orc, deferred):
             compare(res.record(i), orc.sw(reg[i], qo[i], 200, -1, -10, -1), i)
     finally:
         A.set_tuning(before["waves_per_cu"], before["allow_int16"], before["walk_deferred"])
+
+
+def test_autotune_keeps_results(A, orc):
+    """The autotuner only changes speed: after it picks a setting, results
+    are still those of the oracle."""
+    from swift_amd import synth
+    before = A.get_tuning()
+    try:
+        qs, rs = synth.c3_pairs(2000, seed=23)
+        b = A.PairBatch(qs, rs)
+        t = b.autotune(reps=1)
+        assert len(t["table"]) >= 10 and t["ms"] > 0
+        assert A.get_tuning()["walk_deferred"] == t["walk_deferred"]
+        b.run()
+        res = b.results()
+        sc, st, en, al, qc, rc = orc.sw_batch(qs, rs, nthreads=8)
+        assert np.array_equal(res.score, sc) and np.array_equal(res.start, st)
+        assert res.qry_cons == qc and res.ref_cons == rc
+    finally:
+        A.set_tuning(before["waves_per_cu"], before["allow_int16"], before["walk_deferred"])
