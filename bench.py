@@ -262,6 +262,69 @@ def c5_leg(index, seqs, n_requests, seed=20261205):
     return res
 
 
+def c4_leg(n_reads, batch, parity_reads):
+    """Genome scale: 3.1 Gbp / 25 sequences (GC 41 %, 45 % interspersed
+    repeats), k=12 index resident in HBM, 10M reads of 150 bp at 1 %
+    substitutions streamed through the device pipeline in batches.  Reports
+    index build time and reads/s of the timed batches (upload of each batch
+    included); optionally checks `parity_reads` reads against the oracle."""
+    from swift_amd import search, synth
+    from swift_amd.device import synchronize
+    t0 = time.perf_counter()
+    g, off, lens = synth.genome_c4()
+    log(f"[bench] C4 genome {len(g):.3e} bp in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    ix = search.Index.from_packed(g, off, lens, 12, 1000)
+    build_s = time.perf_counter() - t0
+    info = ix.info()
+    log(f"[bench] C4 index: {info} in {build_s:.2f} s")
+    rng = np.random.default_rng(20261204)
+    buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, min(batch, n_reads), 150, 0.01)
+    hits = ix.map_batch(search.ReadBatch.from_packed(buf, ro, rl, 150), 3).nhits
+    hit_stats = {"mean": round(float(hits.mean()), 1), "p50": int(np.percentile(hits, 50)),
+                 "p99": int(np.percentile(hits, 99)), "max": int(hits.max()),
+                 "frac_over_1024": round(float((hits > 1024).mean()), 4),
+                 "frac_over_16384": round(float((hits > 16384).mean()), 5)}
+    log(f"[bench] C4 hits per strand: {hit_stats}")
+    out = search.RecordBuffers(2 * batch * 3, 150)
+    dt, nrec, done = 0.0, 0, 0
+    while done < n_reads:
+        nb = min(batch, n_reads - done)
+        buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, nb, 150, 0.01)
+        synchronize()
+        t0 = time.perf_counter()
+        rb = search.ReadBatch.from_packed(buf, ro, rl, 150)
+        ix.align_batch(rb, out, 3)
+        dt += time.perf_counter() - t0
+        nrec += int(out.bufs["nrecords"].download(np.int32, 1)[0])
+        done += nb
+        log(f"[bench] C4 {done}/{n_reads} reads, {dt:.2f} s on the device path")
+    res = {"workload": "C4: 3.1 Gbp / 25 sequences (GC 41 %, 45 % repeats of 300-6000 bp at "
+                       f"10-20 % divergence), {n_reads} reads of 150 bp, 1 % substitutions, "
+                       "k=12, thr 1000, maxHits 3",
+           "index_build_s": round(build_s, 2), "kept_tuples": info["total"],
+           "largest_bucket": info["max_bucket"], "reads_per_s": round(n_reads / dt, 1),
+           "records": nrec, "batch_reads": batch, "hits_per_strand": hit_stats}
+    if parity_reads > 0:
+        from oracle import oracle as O
+        buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, parity_reads, 150, 0.01)
+        reads = [buf[o:o + 150].tobytes() for o in ro]
+        seqs = [g[o:o + n].tobytes() for o, n in zip(off, lens)]
+        t0 = time.perf_counter()
+        want = O.pipeline(O.Index(seqs, 12, 1000), reads, 3)
+        got = ix.align_reads(reads, 3)
+        keys = (("read", "read"), ("strand", "strand"), ("ref_idx", "refIdx"),
+                ("ref_pos", "refPos"), ("score", "score"), ("start", "start"), ("end", "end"),
+                ("alen", "len"), ("keep", "keep"), ("qry_cons", "qry_cons"),
+                ("ref_cons", "ref_cons"))
+        same = got.n == len(want) and all(
+            got.record(i)[a] == w[b] for i, w in enumerate(want) for a, b in keys)
+        res["parity_vs_oracle"] = {"reads": parity_reads, "records": len(want),
+                                   "identical": bool(same),
+                                   "oracle_s": round(time.perf_counter() - t0, 1)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +340,10 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-requests", type=int, default=400)
+    ap.add_argument("--c4-only", action="store_true", help="only the genome-scale C4 leg")
+    ap.add_argument("--c4-reads", type=int, default=10_000_000)
+    ap.add_argument("--c4-batch", type=int, default=250_000)
+    ap.add_argument("--c4-parity", type=int, default=0, metavar="READS")
     ap.add_argument("--c5-only", action="store_true", help="only the C5 serving leg on the C2 "
                     "index (for profiling)")
     ap.add_argument("--c2-only", action="store_true", help="only the C2 pipeline leg (no JSON "
@@ -305,6 +372,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.c4_only:
+        print(json.dumps(c4_leg(args.c4_reads, args.c4_batch, args.c4_parity)), flush=True)
+        return
     if args.c5_only:
         from swift_amd import search, synth
         seqs, _, _ = synth.c2_workload(n_reads=10)

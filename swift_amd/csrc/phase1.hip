@@ -451,15 +451,92 @@ __global__ void __launch_bounds__(64) map_strands(MapArgs a) {
     emit_best(a, sidx, keys, total);
 }
 
-// listed strands: one 256-thread workgroup each, keys in a scratch slot
+// Strands past one tier's capacity are sorted by a workgroup of THREADS
+// threads with CAP keys in LDS: thread t gathers query offsets t, t + THREADS,
+// ...; a workgroup scan of the per-thread hit counts lays the hits out.
+// Strands past CAP hits are listed for the next tier (`next_list`).  The
+// tiers: 256 threads / 4096 keys (32 KB LDS, several workgroups per CU),
+// 1024 threads / 16384 keys (128 KB).
+template <int THREADS, int CAP>
+__global__ void __launch_bounds__(THREADS)
+map_strands_wg(MapArgs a, const int32_t *list, const int32_t *list_count, int32_t *next_list,
+               int32_t *next_count) {
+    constexpr int NW = THREADS / 64;
+    constexpr int PER = (kMaxMapRead + THREADS) / THREADS;  // offsets per thread
+    __shared__ uint64_t keys[CAP];
+    __shared__ unsigned char qb[kMaxMapRead + 1];
+    __shared__ int32_t wsum[NW];
+    const int n = *list_count;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int sidx = list[it];
+        int L;
+        load_strand(a, sidx, qb, &L);
+        __syncthreads();
+        const int nq = L - a.ix.k + 1;
+        int c[PER], st[PER], v = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int o = t + u * THREADS;
+            c[u] = 0;
+            st[u] = 0;
+            if (o < nq) {
+                const int2 b = a.ix.bucket[kmer_hash(qb + o, a.ix.k)];
+                st[u] = b.x;
+                c[u] = b.y;
+            }
+            v += c[u];
+        }
+        const int incl = wave_incl_scan(v);
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const int x = wsum[w];
+            before += w < wv ? x : 0;
+            total += x;
+        }
+        if (total > CAP) {
+            if (t == 0) next_list[atomicAdd(next_count, 1)] = sidx;
+            __syncthreads();
+            continue;
+        }
+        int at = before + incl - v;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int o = t + u * THREADS;
+            const uint64_t *e = a.ix.entry + st[u];
+            int j = 0;
+            for (; j + 4 <= c[u]; j += 4) {  // four independent loads in flight
+                const uint64_t e0 = e[j], e1 = e[j + 1], e2 = e[j + 2], e3 = e[j + 3];
+                keys[at + j] = hit_key(e0, o);
+                keys[at + j + 1] = hit_key(e1, o);
+                keys[at + j + 2] = hit_key(e2, o);
+                keys[at + j + 3] = hit_key(e3, o);
+            }
+            for (; j < c[u]; ++j) keys[at + j] = hit_key(e[j], o);
+            at += c[u];
+        }
+        const int n2 = next_pow2(total);
+        for (int i = total + t; i < n2; i += THREADS) keys[i] = ~0ull;
+        __syncthreads();
+        bitonic<false>(keys, n2, t, THREADS, [] { __syncthreads(); });
+        if (t < 64) emit_best(a, sidx, keys, total);
+        __syncthreads();
+    }
+}
+
+// strands past kMidCap hits: one 256-thread workgroup each, keys in a
+// power-of-two global scratch slot
 __global__ void __launch_bounds__(256) map_strands_big(MapArgs a) {
     __shared__ unsigned char qb[kMaxMapRead + 1];
     __shared__ int32_t ex[64], stt[64];
     __shared__ int32_t sh_total, sh_ok;
-    const int n = *a.ovl_count;
+    const int n = *a.big_count;
     uint64_t *kbuf = a.scratch + (int64_t)blockIdx.x * a.scratch_cap;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const int sidx = a.ovl_list[it];
+        const int sidx = a.big_list[it];
         int L;
         load_strand(a, sidx, qb, &L);
         __syncthreads();
@@ -683,9 +760,15 @@ hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildS
     return hipGetLastError();
 }
 
-hipError_t launch_map(const MapArgs &a, hipStream_t st) {
+hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
     if (a.rd.nreads <= 0) return hipSuccess;
     hipLaunchKernelGGL(map_strands, dim3(2 * a.rd.nreads), dim3(64), 0, st, a);
+    if (mid_blocks > 0) {
+        hipLaunchKernelGGL((map_strands_wg<256, kMidCap1>), dim3(5 * mid_blocks), dim3(256), 0,
+                           st, a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);
+        hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), dim3(mid_blocks), dim3(1024), 0, st,
+                           a, a.mid_list, a.mid_count, a.big_list, a.big_count);
+    }
     if (a.scratch_slots > 0)
         hipLaunchKernelGGL(map_strands_big, dim3(a.scratch_slots), dim3(256), 0, st, a);
     return hipGetLastError();

@@ -14,7 +14,9 @@ constexpr int kMaxSeeds = 15;          // k <= 15 keeps 4^k buckets < 2^30
 constexpr int kMaxIndexSeqs = 127;     // refIdx is a signed char in the reference
 constexpr int kMaxMapRead = 2047;      // query offsets fit 11 bits
 constexpr int64_t kShiftBias = 2048;   // shift >= -(kMaxMapRead - 1)
-constexpr int kMapCap = 1024;          // hits per strand sorted in LDS
+constexpr int kMapCap = 1024;          // hits per strand sorted in LDS by one wave
+constexpr int kMidCap1 = 4096;         // ... by one 256-thread workgroup (32 KB LDS)
+constexpr int kMidCap = 16384;         // ... by one 1024-thread workgroup (128 KB LDS)
 constexpr int kSmallBucket = 32;       // buckets sorted by one thread
 constexpr int kWaveBucket = 1024;      // buckets sorted by one wave in LDS
 constexpr int kScanChunk = 4096;       // hashes per scan workgroup
@@ -75,7 +77,10 @@ struct MapArgs {
     uint64_t *scratch;        // overflow strands: slots of scratch_cap keys
     int64_t scratch_cap;
     int scratch_slots;
-    int32_t *ovl_list, *ovl_count;
+    int32_t *ovl_list, *ovl_count;   // strands past kMapCap hits
+    int32_t *mid_list, *mid_count;   // strands past kMidCap1 hits
+    int32_t *big_list, *big_count;   // strands past kMidCap hits
+    int mid_blocks;                  // workgroups of map_strands_mid (0: none needed)
     int32_t *error;           // bit 0: a strand exceeded scratch_cap, bit 1: a read
                               // longer than max_len
 };
@@ -98,7 +103,9 @@ hipError_t launch_index_build(const IndexDev &ix, int64_t ntup, const IndexBuild
                               hipStream_t st);
 hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,
                              hipStream_t st);
-hipError_t launch_map(const MapArgs &a, hipStream_t st);
+// mid_blocks: persistent workgroups of map_strands_mid (0 when no strand can
+// pass kMapCap hits)
+hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks);
 hipError_t launch_records(const MapArgs &m, const PairsDev &p, int before, int win,
                           hipStream_t st);
 hipError_t launch_choose_best(const PairsDev &p, int nreads, const int32_t *score,

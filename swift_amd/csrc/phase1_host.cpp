@@ -288,12 +288,14 @@ static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     if (want_strands && (rc = ensure(w.strand_seq, (size_t)ns * qstride + 64))) return rc;
     int slots = 0;
     int64_t cap = 0;
-    if (bound > kMapCap) {
+    if (bound > kMidCap) {
         // the workgroup sort pads a strand's keys to a power of two in its slot
         cap = next_pow2_64(bound);
-        slots = (int)std::max<int64_t>(1, std::min<int64_t>(64, (((int64_t)256 << 20) / (cap * 8))));
+        slots = (int)std::max<int64_t>(1, std::min<int64_t>(512, (((int64_t)512 << 20) / (cap * 8))));
         if ((rc = ensure(w.scratch, (size_t)slots * cap * 8))) return rc;
     }
+    if ((rc = ensure(w.big_list, 4 * ((size_t)ns + 16)))) return rc;
+    if ((rc = ensure(w.mid_list, 4 * ((size_t)ns + 16)))) return rc;
     m->ix = ix.dev();
     m->rd = rd;
     m->out = MapOut{w.ref.as<int8_t>(), w.shift.as<int32_t>(), w.pos.as<int32_t>(),
@@ -308,12 +310,18 @@ static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     m->ovl_list = w.ovl_list.as<int32_t>();
     m->ovl_count = w.flags.as<int32_t>();
     m->error = w.flags.as<int32_t>() + 1;
+    m->big_list = w.big_list.as<int32_t>();
+    m->big_count = w.flags.as<int32_t>() + 2;
+    m->mid_list = w.mid_list.as<int32_t>();
+    m->mid_count = w.flags.as<int32_t>() + 3;
+    m->mid_blocks = bound > kMapCap ? 2 * 256 : 0;
     return kOk;
 }
 
 static int run_map(const MapArgs &m, hipStream_t st) {
-    SWIFT_HIP_TRY(launch_fill(m.ovl_count, 2, 0, st));  // overflow count, error flags
-    SWIFT_HIP_TRY(launch_map(m, st));
+    // overflow count, error flags, big- and mid-strand counts
+    SWIFT_HIP_TRY(launch_fill(m.ovl_count, 4, 0, st));
+    SWIFT_HIP_TRY(launch_map(m, st, m.mid_blocks));
     return kOk;
 }
 

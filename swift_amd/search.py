@@ -49,6 +49,17 @@ class Index:
         return cls(h, k, thr)
 
     @classmethod
+    def from_packed(cls, buf, off, lens, k=12, thr=1000) -> "Index":
+        """Sequences s = buf[off[s] .. off[s] + lens[s]) of a uint8 array."""
+        buf = np.ascontiguousarray(buf, np.uint8)
+        off = np.ascontiguousarray(off, np.int64)
+        lens = np.ascontiguousarray(lens, np.int64)
+        h = C.c_void_p()
+        check(lib().swift_index_build(_p(buf, C.c_char), _p(off, C.c_int64), _p(lens, C.c_int64),
+                                      len(off), k, thr, C.byref(h)))
+        return cls(h, k, thr)
+
+    @classmethod
     def from_fasta(cls, path: str, k=12, thr=1000) -> "Index":
         h = C.c_void_p()
         check(lib().swift_index_build_fasta(str(path).encode(), k, thr, C.byref(h)))
@@ -127,10 +138,20 @@ class Index:
 class ReadBatch:
     """Reads resident on the device."""
 
+    @classmethod
+    def from_packed(cls, buf, off, lens, max_read_len=None) -> "ReadBatch":
+        self = cls.__new__(cls)
+        self._setup(np.ascontiguousarray(buf, np.uint8), np.ascontiguousarray(off, np.int64),
+                    np.ascontiguousarray(lens, np.int32), max_read_len)
+        return self
+
     def __init__(self, reads, max_read_len=None):
         reads = [bytes(r) for r in reads]
         buf, off, lens = pack(reads)
-        self.n = len(reads)
+        self._setup(buf, off, lens, max_read_len)
+
+    def _setup(self, buf, off, lens, max_read_len):
+        self.n = len(lens)
         self.max_len = int(max_read_len if max_read_len is not None
                            else (lens.max() if self.n else 0))
         self.seq = DeviceBuffer.from_array(buf)

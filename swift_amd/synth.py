@@ -208,3 +208,66 @@ def to_fasta(seqs, width=50, names=None) -> bytes:
 
 def reads_fasta(reads) -> bytes:
     return b"".join(b">r%09d\n%s\n" % (i, r) for i, r in enumerate(reads))
+
+
+# ---------------------------------------------------------------- config 4
+def genome_c4(total=3_100_000_000, nseq=25, seed=None, gc=0.41, repeat_frac=0.45, nfam=1000,
+              chunk=1 << 27):
+    """25 sequences totalling `total` bp (GC 41 %); 45 % of the bases are
+    copies of repeat families (consensus length U[300, 6000]) at U[10 %, 20 %]
+    substitution divergence, dropped at uniform positions.  Returns (packed
+    uint8 buffer, int64 offsets, int64 lengths).  Generated in chunks so the
+    peak memory stays near the genome's own size."""
+    rng = np.random.default_rng(seed_for(4) if seed is None else seed)
+    g = np.empty(total, np.uint8)
+    lut = np.empty(1000, np.uint8)
+    at = int(round(500 * (1 - gc)))
+    lut[:at], lut[at:500] = ord("A"), ord("C")
+    lut[500:1000 - at], lut[1000 - at:] = ord("G"), ord("T")
+    for c0 in range(0, total, chunk):
+        n = min(chunk, total - c0)
+        g[c0:c0 + n] = lut[rng.integers(0, 1000, n, dtype=np.int16)]
+    fams = []
+    for _ in range(nfam):
+        L = int(rng.integers(300, 6001))
+        fams.append(lut[rng.integers(0, 1000, L, dtype=np.int16)])
+    covered, target = 0, int(repeat_frac * total)
+    while covered < target:
+        f = fams[int(rng.integers(0, nfam))]
+        div = rng.uniform(0.10, 0.20)
+        cp = f.copy()
+        m = rng.random(len(cp)) < div
+        if m.any():
+            code = np.searchsorted(BASES, cp[m])
+            cp[m] = BASES[(code + rng.integers(1, 4, m.sum())) % 4]
+        p = int(rng.integers(0, total - len(cp)))
+        g[p:p + len(cp)] = cp
+        covered += len(cp)
+    cuts = np.sort(rng.choice(np.arange(1, nseq * 1000), nseq - 1, replace=False))
+    w = np.diff(np.concatenate([[0], cuts, [nseq * 1000]])).astype(np.float64)
+    lens = (w / w.sum() * total).astype(np.int64)
+    lens[-1] += total - lens.sum()
+    off = np.zeros(nseq, np.int64)
+    off[1:] = np.cumsum(lens)[:-1]
+    return g, off, lens
+
+
+def sample_reads_fast(rng, genome, off, lens, n, length, sub):
+    """n reads of `length` bp at uniform positions inside one sequence each,
+    uniform strand, substitutions at rate `sub` (vectorised; no indels).
+    Returns (packed uint8 buffer, int64 offsets, int32 lengths)."""
+    w = (lens - length).astype(np.float64)
+    which = rng.choice(len(lens), size=n, p=w / w.sum())
+    pos = off[which] + (rng.random(n) * (lens[which] - length)).astype(np.int64)
+    out = genome[pos[:, None] + np.arange(length)[None, :]]
+    m = rng.random(out.shape) < sub
+    if m.any():
+        code = np.searchsorted(BASES, out[m])
+        out[m] = BASES[(code + rng.integers(1, 4, m.sum())) % 4]
+    rc = rng.random(n) < 0.5
+    if rc.any():
+        lut = np.full(256, ord("A"), np.uint8)
+        lut[ord("A")], lut[ord("C")], lut[ord("G")] = ord("T"), ord("G"), ord("C")
+        out[rc] = lut[out[rc][:, ::-1]]
+    return (out.reshape(-1), np.arange(n, dtype=np.int64) * length,
+            np.full(n, length, np.int32))

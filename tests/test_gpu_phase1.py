@@ -129,14 +129,20 @@ def test_map_reads_vs_oracle(S, orc, max_hits):
 
 
 def test_map_reads_overflow_strands(S, orc):
-    """Strands with more hits than the LDS sort holds take the scratch path."""
+    """Strands past the wave's LDS sort (1024 hits) take the workgroup LDS
+    sort, those past 16384 hits the global scratch path; all three tiers in
+    one call."""
     rnd = random.Random(4)
     unit = b"ACGTTGCAAC"
     seqs = [unit * 2000, bytes(rnd.choice(b"ACGT") for _ in range(20000)) + unit * 300]
-    reads = [unit * 20, (unit * 30)[3:250], seqs[1][100:300], unit[:5] * 40]
+    tail = bytes(rnd.choice(b"ACGT") for _ in range(90))
+    reads = [unit * 20, (unit * 30)[3:250], seqs[1][100:300], unit[:5] * 40, unit + tail,
+             unit * 2 + tail, unit * 5 + tail, unit * 8 + tail]
     ix = S.Index.from_sequences(seqs, 10, 100000)
     got = ix.map_reads(reads, 3)
-    assert got.nhits.max() > 1024
+    h = got.nhits
+    for lo, hi in ((0, 1024), (1024, 4096), (4096, 16384), (16384, 1 << 30)):
+        assert ((h > lo) & (h <= hi)).any(), (lo, hi, h)
     check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
 
 
