@@ -527,6 +527,162 @@ map_strands_wg(MapArgs a, const int32_t *list, const int32_t *list_count, int32_
     }
 }
 
+// Hash tier (strands of kMapCap < hits <= kMidCap1 when 32-bit cluster ids
+// fit): no sort.  Each hit is counted into an LDS open-addressing table keyed
+// by its cluster id (count and smallest query offset packed in one word,
+// updated by compare-and-swap); workgroup reductions then give the biggest
+// count, the number of clusters of that count, and the first max_hits such
+// clusters in cluster-id order, i.e. the (refIdx, shift) order of
+// findBiggestClusters with the first element of each cluster.
+__device__ __forceinline__ uint32_t cid_slot(uint32_t cid) {
+    return (cid * 2654435761u) >> (32 - 13);  // log2(kHashSlots) = 13
+}
+
+template <int THREADS>
+__device__ __forceinline__ uint64_t block_min_u64(uint64_t v, uint64_t *red) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        uint64_t o = __shfl_xor(v, s, 64);
+        v = o < v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int w = 0; w < THREADS / 64; ++w) m = red[w] < m ? red[w] : m;
+    __syncthreads();
+    return m;
+}
+
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS)
+map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int32_t *next_list,
+                 int32_t *next_count) {
+    constexpr int NW = THREADS / 64;
+    constexpr int PER = (kMaxMapRead + THREADS) / THREADS;
+    constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+    constexpr uint32_t kFresh = 0x0000FFFFu;  // count 0, smallest offset 0xFFFF
+    __shared__ uint32_t tkey[kHashSlots], tval[kHashSlots];
+    __shared__ unsigned char qb[kMaxMapRead + 1];
+    __shared__ int32_t wsum[NW];
+    __shared__ uint64_t red[NW];
+    const int n = *list_count;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i = t; i < kHashSlots; i += THREADS) {
+        tkey[i] = kEmpty;
+        tval[i] = kFresh;
+    }
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int sidx = list[it];
+        int L;
+        load_strand(a, sidx, qb, &L);
+        __syncthreads();
+        const int nq = L - a.ix.k + 1;
+        int c[PER], st[PER], v = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int o = t + u * THREADS;
+            c[u] = 0;
+            st[u] = 0;
+            if (o < nq) {
+                const int2 b = a.ix.bucket[kmer_hash(qb + o, a.ix.k)];
+                st[u] = b.x;
+                c[u] = b.y;
+            }
+            v += c[u];
+        }
+        const int incl = wave_incl_scan(v);
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        int total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) total += wsum[w];
+        if (total > kMidCap1) {
+            if (t == 0) next_list[atomicAdd(next_count, 1)] = sidx;
+            __syncthreads();
+            continue;
+        }
+        // count every hit into the table
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int o = t + u * THREADS;
+            const uint64_t *e = a.ix.entry + st[u];
+            for (int j = 0; j < c[u]; ++j) {
+                const uint64_t en = e[j];
+                const uint32_t cid = a.ix.cid0[(int)(en >> 32)] +
+                                     (uint32_t)((int64_t)(uint32_t)en - o + kShiftBias);
+                uint32_t h = cid_slot(cid);
+                for (;;) {
+                    const uint32_t old = atomicCAS(&tkey[h], kEmpty, cid);
+                    if (old == kEmpty || old == cid) break;
+                    h = (h + 1) & (kHashSlots - 1);
+                }
+                uint32_t cur = tval[h], seen;
+                do {
+                    seen = cur;
+                    const uint32_t cnt = (seen >> 16) + 1;
+                    const uint32_t mo = min(seen & 0xFFFFu, (uint32_t)o);
+                    cur = atomicCAS(&tval[h], seen, (cnt << 16) | mo);
+                } while (cur != seen);
+            }
+        }
+        __syncthreads();
+        // biggest count and the number of clusters that reach it
+        uint32_t best = 0;
+        for (int i = t; i < kHashSlots; i += THREADS) best = max(best, tval[i] >> 16);
+        best = (uint32_t)(~block_min_u64<THREADS>(~(uint64_t)best, red));
+        int nbig = 0;
+        for (int i = t; i < kHashSlots; i += THREADS) nbig += (tval[i] >> 16) == best;
+        const int nbig_w = wave_incl_scan(nbig);
+        if (lane == 63) wsum[wv] = nbig_w;
+        __syncthreads();
+        int nbig_all = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) nbig_all += wsum[w];
+        // the first max_hits of them by cluster id
+        const int take = min(nbig_all, a.max_hits);
+        const int64_t ob = (int64_t)sidx * a.out.stride;
+        uint64_t last = 0;
+        for (int r = 0; r < take; ++r) {
+            uint64_t mine = ~0ull;
+            for (int i = t; i < kHashSlots; i += THREADS) {
+                if ((tval[i] >> 16) != best) continue;
+                const uint64_t k = ((uint64_t)tkey[i] << 16) | (tval[i] & 0xFFFFu);
+                if ((r == 0 || k > last) && k < mine) mine = k;
+            }
+            const uint64_t k = block_min_u64<THREADS>(mine, red);
+            last = k;
+            if (t == 0) {
+                const uint32_t cid = (uint32_t)(k >> 16);
+                int ref = 0;  // last sequence whose cid0 <= cid
+                for (int lo = 0, hi = a.ix.nseq - 1; lo <= hi;) {
+                    const int mid = (lo + hi) >> 1;
+                    if (a.ix.cid0[mid] <= cid) {
+                        ref = mid;
+                        lo = mid + 1;
+                    } else {
+                        hi = mid - 1;
+                    }
+                }
+                const int sh = (int)((int64_t)(cid - a.ix.cid0[ref]) - kShiftBias);
+                a.out.ref[ob + r] = (int8_t)ref;
+                a.out.shift[ob + r] = sh;
+                a.out.pos[ob + r] = sh + (int)(k & 0xFFFFu);
+            }
+        }
+        if (t == 0) {
+            a.out.nbest[sidx] = take;
+            a.out.nbig[sidx] = nbig_all;
+        }
+        __syncthreads();
+        for (int i = t; i < kHashSlots; i += THREADS) {
+            tkey[i] = kEmpty;
+            tval[i] = kFresh;
+        }
+        __syncthreads();
+    }
+}
+
 // strands past kMidCap hits: one 256-thread workgroup each, keys in a
 // power-of-two global scratch slot
 __global__ void __launch_bounds__(256) map_strands_big(MapArgs a) {
@@ -764,8 +920,12 @@ hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
     if (a.rd.nreads <= 0) return hipSuccess;
     hipLaunchKernelGGL(map_strands, dim3(2 * a.rd.nreads), dim3(64), 0, st, a);
     if (mid_blocks > 0) {
-        hipLaunchKernelGGL((map_strands_wg<256, kMidCap1>), dim3(5 * mid_blocks), dim3(256), 0,
-                           st, a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);
+        if (a.ix.cid_ok)
+            hipLaunchKernelGGL((map_strands_hash<512>), dim3(2 * mid_blocks), dim3(512), 0, st,
+                               a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);
+        else
+            hipLaunchKernelGGL((map_strands_wg<256, kMidCap1>), dim3(5 * mid_blocks), dim3(256),
+                               0, st, a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);
         hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), dim3(mid_blocks), dim3(1024), 0, st,
                            a, a.mid_list, a.mid_count, a.big_list, a.big_count);
     }

@@ -16,6 +16,8 @@ constexpr int kMaxMapRead = 2047;      // query offsets fit 11 bits
 constexpr int64_t kShiftBias = 2048;   // shift >= -(kMaxMapRead - 1)
 constexpr int kMapCap = 1024;          // hits per strand sorted in LDS by one wave
 constexpr int kMidCap1 = 4096;         // ... by one 256-thread workgroup (32 KB LDS)
+constexpr int kHashSlots = 8192;       // cluster table of the hash tier (64 KB LDS):
+                                       // strands of <= kMidCap1 hits
 constexpr int kMidCap = 16384;         // ... by one 1024-thread workgroup (128 KB LDS)
 constexpr int kSmallBucket = 32;       // buckets sorted by one thread
 constexpr int kWaveBucket = 1024;      // buckets sorted by one wave in LDS
@@ -36,6 +38,11 @@ struct IndexDev {
     int32_t *cursor;          // [nhash] fill cursor
     int2 *bucket;             // [nhash] (start, actual) for the mapper
     uint64_t *entry;          // [total + 1] refIdx << 32 | refPos, [0] sentinel
+    // 32-bit cluster ids: cid = cid0[refIdx] + shift + kShiftBias, with
+    // cid0[s] = sum over r < s of (len[r] + 2 * kShiftBias); increasing in
+    // (refIdx, shift).  cid_ok = 0 when the sum does not fit (> 4 Gbp).
+    const uint32_t *cid0;
+    int cid_ok;
 };
 
 struct IndexBuildScratch {

@@ -164,6 +164,19 @@ static int index_build(const char *bases, const int64_t *off, const int64_t *len
         if (len[s] > 0)
             SWIFT_HIP_TRY(hipMemcpy(ix->seq.as<char>() + ix->h_off[s], bases + off[s], (size_t)len[s],
                                     hipMemcpyHostToDevice));
+    {
+        // cluster-id bases: cid = cid0[s] + shift + kShiftBias stays below
+        // 2^32 - 1 (the empty-slot marker of the hash tier)
+        std::vector<uint32_t> c0(std::max(nseq, 1), 0);
+        uint64_t acc = 0;
+        for (int s = 0; s < nseq; ++s) {
+            c0[s] = (uint32_t)std::min<uint64_t>(acc, 0xFFFFFFFFull);
+            acc += (uint64_t)len[s] + 2 * kShiftBias;
+        }
+        ix->cid_ok = acc < 0xFFFFFFFFull ? 1 : 0;
+        if ((rc = ix->cid0.alloc(4 * c0.size()))) return rc;
+        SWIFT_HIP_TRY(hipMemcpy(ix->cid0.ptr, c0.data(), 4 * c0.size(), hipMemcpyHostToDevice));
+    }
     if ((rc = ix->seq_off.alloc(8 * (size_t)std::max(nseq, 1)))) return rc;
     if ((rc = ix->seq_len.alloc(8 * (size_t)std::max(nseq, 1)))) return rc;
     if ((rc = ix->tup0.alloc(8 * (size_t)std::max(nseq, 1)))) return rc;

@@ -37,7 +37,8 @@ struct DevArr {
 struct IndexObj {
     int k = 0, thr = 0, nseq = 0, max_actual = 0;
     int64_t nhash = 0, ntup = 0, total = 0;
-    DevArr seq, seq_off, seq_len, tup0, count, start, actual, cursor, bucket, entry;
+    DevArr seq, seq_off, seq_len, tup0, count, start, actual, cursor, bucket, entry, cid0;
+    int cid_ok = 0;
     std::vector<int64_t> h_off, h_len;
     std::vector<std::string> names;
 
@@ -57,6 +58,8 @@ struct IndexObj {
         d.cursor = cursor.as<int32_t>();
         d.bucket = bucket.as<int2>();
         d.entry = entry.as<uint64_t>();
+        d.cid0 = cid0.as<const uint32_t>();
+        d.cid_ok = cid_ok;
         return d;
     }
 };
