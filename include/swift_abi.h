@@ -24,6 +24,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -267,6 +268,31 @@ int swift_pipeline_graph_create(const swift_index *ix, const char *reads_d, cons
 int swift_pipeline_graph_launch(swift_graph *g, void *stream);
 int swift_pipeline_graph_check(swift_graph *g);
 int swift_pipeline_graph_free(swift_graph *g);
+
+/* ======================================================== output formats
+ * Drop-ins for the reference's writers (src/output.h:29-55), same output
+ * byte for byte: names at stride MAX_SEQ_NAME_LENGTH (50), consensus strings
+ * in walk order at stride combinedLength; printOutput skips score < 0. */
+void printOutput(int numQueries, char *queryName, char *refName, float *score, int *refDistance,
+                 int *alignStart, int *alignEnd, int *alignLength, int combinedLength,
+                 char *queryConsensus, char *refConsensus, FILE *outputFilePtr);
+void printOutputSam(int numQueries, char *queryName, char *refName, float *score,
+                    int *alignStart, int *alignEnd, int *alignLength, int combinedLength,
+                    char *queryConsensus, char *refConsensus, FILE *outputFilePtr);
+void printCigar(char *queryConsensus, char *refConsensus, int alignLength, FILE *outputFilePtr);
+
+/* CIGAR strings of device records i < *rec->nrecords (printCigar's rule) at
+ * cigar_d + i * cigar_stride (NUL terminated, cigar_stride >= 2 *
+ * cons_stride), lengths in cigar_len_d; max_records bounds the count. */
+int swift_records_cigar(const swift_records *rec, int max_records, char *cigar_d,
+                        int64_t cigar_stride, int32_t *cigar_len_d, void *stream);
+/* Writes pipeline records (host copies: a swift_records of host pointers) to
+ * `path` as printOutput text (format 0) or SAM (format 1).  Record i is named
+ * qry_names[read[i]] (+ '*' on the reverse strand) against the FASTA name of
+ * its index sequence; only_kept writes the records the best-hit choice
+ * keeps. */
+int swift_write_records(const char *path, int format, int only_kept, const swift_records *rec,
+                        int nrec, const char *const *qry_names, const swift_index *ix);
 
 /* ======================================================== serving loop
  * No reference counterpart (the reference is a batch command-line tool).
