@@ -195,7 +195,12 @@ struct Sweep {
         for (int r = 0; r < R; ++r) {
             uint32_t f = sub_flag<PK, POL>(q[r], rr);
             uint32_t dk = vmad<PK>(f, c_dsub, diag);
-            uint32_t K = vmax<PK>(vmax<PK>(dk, eu), vmax<PK>(el[r], 0u));
+            // the diagonal, left and zero candidates do not depend on the row
+            // above's up exit: fold them first so eu (the serial row-to-row
+            // chain) enters last (max is order-free: the direction code in the
+            // key's low bits breaks ties)
+            const uint32_t pre = opaque(vmax<PK>(vmax<PK>(dk, el[r]), 0u));
+            uint32_t K = vmax<PK>(pre, eu);
             uint32_t h4 = K & H4MASK;
             uint32_t c = K & CMASK;
             acc[r / KPW] |= c << (2 * (r % KPW));
@@ -511,7 +516,7 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
             uint32_t code = POL == kOneHot ? kRefOther : 0u;
             if (j < rl[h]) {
                 unsigned char c = (unsigned char)a.rseq[ro[h] + j];
-                code = POL == kOneHot ? ref_index(c) : byte_rcode(c);
+                code = POL == kOneHot ? ref_index_bf(c) : byte_rcode(c);
                 if (POL == kOneHot && code == kRefOther) exotic |= 1u << h;
             }
             v |= code << (16 * h);
@@ -533,7 +538,7 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
         for (int h = 0; h < PPG; ++h) {
             if (row < ql[h]) {
                 unsigned char c = (unsigned char)a.qseq[qo[h] + row];
-                uint32_t code = POL == kOneHot ? query_mask(c) : byte_qcode(c);
+                uint32_t code = POL == kOneHot ? query_mask_bf(c) : byte_qcode(c);
                 v |= code << (16 * h);
             }
         }

@@ -85,7 +85,7 @@ __host__ __device__ inline uint32_t query_code(unsigned char c) {
 // query byte a bit mask over the classes it matches, so
 // match(q, r) <=> (query_mask(q) >> ref_index(r)) & 1.
 constexpr uint32_t kRefOther = 5;
-__host__ __device__ inline uint32_t ref_index(unsigned char c) {
+__host__ __device__ constexpr uint32_t ref_index(unsigned char c) {
     switch (c) {
     case 'A': case 'a': return 0;
     case 'C': case 'c': return 1;
@@ -95,7 +95,7 @@ __host__ __device__ inline uint32_t ref_index(unsigned char c) {
     default: return kRefOther;
     }
 }
-__host__ __device__ inline uint32_t query_mask(unsigned char c) {
+__host__ __device__ constexpr uint32_t query_mask(unsigned char c) {
     switch (c) {
     case 'A': return 0x11;
     case 'C': return 0x12;
@@ -105,6 +105,30 @@ __host__ __device__ inline uint32_t query_mask(unsigned char c) {
     default: return 0x10;  // only a reference N matches
     }
 }
+
+// Branch-free forms of the two classifiers for the kernels (a switch over
+// bytes compiles to chains of compares under exec masks).  (b >> 1) & 7
+// separates A, C, G, T, N (and a, c, g, t, n): slots 0, 1, 3, 2, 7; a byte
+// is the slot's letter or falls in the "other" class.
+__host__ __device__ constexpr uint32_t ref_index_bf(unsigned char c) {
+    const uint32_t u = c | 0x20u;  // the reference byte, case folded
+    const uint32_t h = (u >> 1) & 7u;
+    const uint32_t letter = (uint32_t)(0x6E00000067746361ull >> (8 * h)) & 0xFFu;
+    return letter == u ? (0x45552310u >> (4 * h)) & 0xFu : kRefOther;
+}
+__host__ __device__ constexpr uint32_t query_mask_bf(unsigned char c) {
+    const uint32_t h = ((uint32_t)c >> 1) & 7u;  // the query byte, case sensitive
+    const uint32_t letter = (uint32_t)(0x4E00000047544341ull >> (8 * h)) & 0xFFu;
+    return letter == c ? (uint32_t)(0x3F00000014181211ull >> (8 * h)) & 0xFFu : 0x10u;
+}
+constexpr bool classifiers_agree() {
+    for (int c = 0; c < 256; ++c)
+        if (ref_index_bf((unsigned char)c) != ref_index((unsigned char)c) ||
+            query_mask_bf((unsigned char)c) != query_mask((unsigned char)c))
+            return false;
+    return true;
+}
+static_assert(classifiers_agree(), "branch-free byte classifiers differ from the tables");
 
 // 0 marks a reference byte the one-hot kernels cannot represent.
 __host__ __device__ inline uint32_t ref_code(unsigned char c) {
