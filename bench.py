@@ -262,7 +262,7 @@ def c5_leg(index, seqs, n_requests, seed=20261205):
     return res
 
 
-def c4_leg(n_reads, batch, parity_reads):
+def c4_leg(n_reads, batch, parity_reads, c5_requests=0):
     """Genome scale: 3.1 Gbp / 25 sequences (GC 41 %, 45 % interspersed
     repeats), k=12 index resident in HBM, 10M reads of 150 bp at 1 %
     substitutions streamed through the device pipeline in batches.  Reports
@@ -322,6 +322,9 @@ def c4_leg(n_reads, batch, parity_reads):
         res["parity_vs_oracle"] = {"reads": parity_reads, "records": len(want),
                                    "identical": bool(same),
                                    "oracle_s": round(time.perf_counter() - t0, 1)}
+    if c5_requests > 0:
+        res["c5"] = c5_leg(ix, [g[o:o + n] for o, n in zip(off, lens)], c5_requests)
+        res["c5"]["index"] = "C4 index (3.1 Gbp / 25 sequences, k=12) resident"
     return res
 
 
@@ -373,7 +376,8 @@ def main():
         sys.exit(2)
 
     if args.c4_only:
-        print(json.dumps(c4_leg(args.c4_reads, args.c4_batch, args.c4_parity)), flush=True)
+        print(json.dumps(c4_leg(args.c4_reads, args.c4_batch, args.c4_parity,
+                                0 if args.no_c5 else args.c5_requests)), flush=True)
         return
     if args.c5_only:
         from swift_amd import search, synth
