@@ -84,7 +84,7 @@ template <bool PK> __device__ __forceinline__ uint32_t vbit(uint32_t k, uint32_t
 // v_bitop3 arithmetic instead of being rewritten into per-half compares and
 // selects (there is no packed select).
 __device__ __forceinline__ uint32_t opaque(uint32_t x) {
-    asm volatile("" : "+v"(x));
+    asm("" : "+v"(x));
     return x;
 }
 
@@ -105,16 +105,14 @@ __device__ __forceinline__ uint32_t byte_rcode(unsigned char c) {
     return (uint32_t)u | (u == 'N' ? 0u : 0xFF00u);
 }
 
-// substitution flag (0/1 per element)
+// substitution flag (0/1 per element); one: 1 per element, a run-time value
 template <bool PK, int POL>
-__device__ __forceinline__ uint32_t sub_flag(uint32_t q, uint32_t r) {
+__device__ __forceinline__ uint32_t sub_flag(uint32_t q, uint32_t r, uint32_t one) {
     if constexpr (POL == kOneHot) {
         // q: per element class mask of the query byte, r: class index of the
         // reference byte; 1 = match
-        uint32_t f;
-        if constexpr (PK) f = as_u(as_us(q) >> as_us(r)) & 0x00010001u;
-        else f = (q >> r) & 1u;
-        return opaque(f);
+        if constexpr (PK) return as_u(as_us(q) >> as_us(r)) & one;
+        else return (q >> r) & one;
     } else {
         static_assert(!PK, "byte policy is int32 only");
         uint32_t x = (q ^ r) & ((q & r) >> 8) & 0xFFu;
@@ -141,7 +139,7 @@ struct Sweep {
     uint32_t bcol[R];     // PK only: first column of the row max (both halves)
     uint32_t eu_last, dr_last, dr_top;
 
-    uint32_t c_dr, c_dsub, c_eo2, c_eo1, c_de;
+    uint32_t c_dr, c_dsub, c_eo2, c_eo1, c_de, c_one, c_sign;
     // PK: byte tables of the gap exits indexed by direction code, laid out
     // for v_perm_b32 (entry of code c at byte 2c+1, sign via selector 8+c)
     uint32_t lut_u_lo, lut_u_hi, lut_l_lo, lut_l_hi;
@@ -158,6 +156,8 @@ struct Sweep {
         c_eo2 = splat<PK>(k.eo2);
         c_eo1 = splat<PK>(k.eo1);
         c_de = splat<PK>(k.de);
+        c_one = splat<PK>(k.one);
+        c_sign = splat<PK>(k.sign);
         // up exit: open unless the cell came from above (code 2); left exit:
         // open unless it came from the left (code 1)
         lut(k.eo2, k.eo2, k.eo2 + k.de, k.eo2, lut_u_lo, lut_u_hi);
@@ -187,24 +187,28 @@ struct Sweep {
         dr_top = dr_next;
         if (MASKED && j < 0) return;  // left of column 1: keep column-0 state
 
-        uint32_t eu = eu_in, diag = diag0;
         uint32_t jj = 0, colterm = 0;
         if constexpr (PK) jj = splat<true>(j);
         else colterm = (uint32_t)(kMaxCols - j);
+        // The diagonal, left and zero candidates depend only on column j-1:
+        // fold them for every row first, so the row-to-row chain through the
+        // up exit eu is just max -> code -> exit per row, with the rows'
+        // independent work free to fill its gaps (max is order-free: the
+        // direction code in the key's low bits breaks ties).
+        uint32_t pre[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            uint32_t f = sub_flag<PK, POL>(q[r], rr);
-            uint32_t dk = vmad<PK>(f, c_dsub, diag);
-            // the diagonal, left and zero candidates do not depend on the row
-            // above's up exit: fold them first so eu (the serial row-to-row
-            // chain) enters last (max is order-free: the direction code in the
-            // key's low bits breaks ties)
-            const uint32_t pre = opaque(vmax<PK>(vmax<PK>(dk, el[r]), 0u));
-            uint32_t K = vmax<PK>(pre, eu);
+            uint32_t f = sub_flag<PK, POL>(q[r], rr, c_one);
+            uint32_t dk = vmad<PK>(f, c_dsub, r == 0 ? diag0 : dr[r - 1]);
+            pre[r] = vmax<PK>(vmax<PK>(dk, el[r]), 0u);
+        }
+        uint32_t eu = eu_in;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            uint32_t K = vmax<PK>(pre[r], eu);
             uint32_t h4 = K & H4MASK;
             uint32_t c = K & CMASK;
             acc[r / KPW] |= c << (2 * (r % KPW));
-            diag = dr[r];
             dr[r] = vadd<PK>(h4, c_dr);
             if constexpr (PK) {
                 // sign-extended per-half gap exit from the byte tables
@@ -218,7 +222,7 @@ struct Sweep {
             if constexpr (PK) {
                 uint32_t hv = MASKED ? (h4 & vmask) : h4;
                 uint32_t d = vsub<true>(best[r], hv);                  // < 0 <=> new strict max
-                uint32_t m = opaque(as_u(as_s(d) >> (v2s){15, 15}));  // 0xFFFF per new max
+                uint32_t m = as_u(as_s(d) >> as_s(c_sign));  // 0xFFFF per new max
                 bcol[r] = (m & jj) | (~m & bcol[r]);
                 best[r] = vmax<true>(best[r], hv);
             } else {

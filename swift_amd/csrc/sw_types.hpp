@@ -13,6 +13,11 @@ struct KeyConsts {
     int eo2;   // 4*gapOpen + 2 (up exit, open)
     int eo1;   // 4*gapOpen + 1 (left exit, open)
     int de;    // 4*(gapExt - gapOpen)
+    // 1 and 15, passed at run time: masks and shifts by them leave the
+    // optimiser no value range to turn 0/1 products and sign-mask merges of
+    // packed halves into per-half compares and selects
+    int one;
+    int sign;
 };
 
 // ONEHOT: flag = match,    dr = 4*mismatch + 3, dsub = 4*(match - mismatch)
@@ -31,6 +36,8 @@ inline KeyConsts make_consts(const Scoring &s, int policy) {
     k.eo2 = 4 * s.gap_open + 2;
     k.eo1 = 4 * s.gap_open + 1;
     k.de = 4 * (s.gap_ext - s.gap_open);
+    k.one = 1;
+    k.sign = 15;
     return k;
 }
 
