@@ -351,10 +351,14 @@ def main():
                     "index (for profiling)")
     ap.add_argument("--c2-only", action="store_true", help="only the C2 pipeline leg (no JSON "
                     "bench line contract; for profiling)")
-    ap.add_argument("--autotune", type=int, default=200000, metavar="PAIRS",
-                    help="autotune the batched path on this many workload pairs first (0: off)")
+    ap.add_argument("--autotune", type=int, default=-1, metavar="PAIRS",
+                    help="autotune the batched path on this many workload pairs first "
+                         "(default -1: the whole workload; 0: off)")
     ap.add_argument("--walk", choices=["default", "fused", "deferred"], default="default",
                     help="traceback walk mode (default: the library's tuned setting)")
+    ap.add_argument("--waves", type=int, default=-1, metavar="N",
+                    help="persistent waves per CU of the row-class kernels (0: occupancy "
+                         "limit; default: the library's tuned setting)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -396,9 +400,11 @@ def main():
     from swift_amd import align, synth
     from swift_amd._lib import lib
     from swift_amd.device import synchronize
-    if args.walk != "default":
+    if args.walk != "default" or args.waves >= 0:
         t = align.get_tuning()
-        align.set_tuning(t["waves_per_cu"], t["allow_int16"], args.walk == "deferred")
+        align.set_tuning(args.waves if args.waves >= 0 else t["waves_per_cu"], t["allow_int16"],
+                         t["walk_deferred"] if args.walk == "default"
+                         else args.walk == "deferred")
 
     t0 = time.perf_counter()
     qbuf, qoff, qlen, rbuf, roff, rlen = synth.c3_pairs_fast(args.pairs)
@@ -406,16 +412,20 @@ def main():
     log(f"[bench] generated {args.pairs} pairs, {cells:.3e} cells in "
         f"{time.perf_counter() - t0:.1f} s")
 
+    batch = align.PairBatch.from_arrays(qbuf, qoff, qlen, rbuf, roff, rlen)
     tuned = None
-    if args.autotune > 0 and args.walk == "default":
-        k = min(args.autotune, args.pairs)
-        sample = align.PairBatch.from_arrays(qbuf, qoff[:k], qlen[:k], rbuf, roff[:k], rlen[:k])
+    n_tune = args.pairs if args.autotune < 0 else min(args.autotune, args.pairs)
+    if n_tune > 0 and args.walk == "default" and args.waves < 0:
+        # the settings are timed on the workload itself: a small prefix does
+        # not rank them the way the full batch does (the walk placement in
+        # particular)
+        sample = batch if n_tune == args.pairs else align.PairBatch.from_arrays(
+            qbuf, qoff[:n_tune], qlen[:n_tune], rbuf, roff[:n_tune], rlen[:n_tune])
         t0 = time.perf_counter()
         tuned = sample.autotune()
-        log(f"[bench] autotune on {k} pairs ({time.perf_counter() - t0:.1f} s): "
+        log(f"[bench] autotune on {n_tune} pairs ({time.perf_counter() - t0:.1f} s): "
             f"{ {x: tuned[x] for x in ('waves_per_cu', 'allow_int16', 'walk_deferred', 'ms')} }")
         del sample
-    batch = align.PairBatch.from_arrays(qbuf, qoff, qlen, rbuf, roff, rlen)
     ms4 = (C.c_float * 4)()
     lib().swift_sw_batch_timing(1, ms4)
     for i in range(args.warmup):
@@ -463,7 +473,7 @@ def main():
         "batch_stats": stats,
     }
     if tuned is not None:
-        out["autotune"] = {"sample_pairs": min(args.autotune, args.pairs),
+        out["autotune"] = {"sample_pairs": n_tune,
                            "chosen": {x: tuned[x] for x in ("waves_per_cu", "allow_int16",
                                                              "walk_deferred")},
                            "candidates_ms": [[r["waves_per_cu"], r["int16"], r["walk_deferred"],

@@ -2,6 +2,7 @@
 // alignSequences drop-in (plain per-pair kernel) and the batched
 // length-binned path (device-side binning, persistent int16 kernel, int32
 // re-run of overflow-suspect pairs, walk-back).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -76,8 +77,11 @@ struct Workspace {
     DevBuf totals;    // int64 total trace words per pass
     int cu_count = 0;
     BatchStats stats{};
-    // side streams the row-class kernels are spread over (fork/join by events)
-    static constexpr int kSide = 4;
+    // Side streams the row-class kernels are spread over (fork/join by
+    // events).  Three: with the caller's stream that is the four hardware
+    // queues a process gets by default, so no two side streams share one
+    // (kernels of streams on one hardware queue run one after another).
+    static constexpr int kSide = 3;
     hipStream_t side[kSide] = {};
     hipEvent_t fork = nullptr, join[kSide] = {};
     bool streams_ready = false;
@@ -173,11 +177,11 @@ int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32) {
         return kErrInvalid;
     }
     // int16 lanes: keys 4*H + 3 within int16, penalties small enough that
-    // the lowest candidate stays above -32768
-    // the packed kernel keeps the gap exits (4*gap + 1 or + 2) in signed
-    // byte tables: gaps must lie in [-31, 31]
-    bool gaps8 = s.gap_open >= -31 && s.gap_open <= 31 && s.gap_ext >= -31 && s.gap_ext <= 31;
-    *force32 = pos > 2000 || neg < -2000 || !gaps8;
+    // the lowest candidate stays above -32768; and mismatch and gap moves
+    // that all lose score (the packed sweep runs its padding columns
+    // unmasked and clamps the left exit at 0)
+    const bool losing = s.mismatch < 0 && s.gap_open <= -1 && s.gap_ext <= -1;
+    *force32 = pos > 2000 || neg < -2000 || !losing;
     return kOk;
 }
 
@@ -254,6 +258,10 @@ static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
     return (t16 * w16 + 2 * t32 * w32) * steps * 64;
 }
 
+// totals layout (int64): [pass] trace words, then from kTotSteps the sweep
+// steps of each (pass, segment)
+constexpr int kTotSteps = 8;
+
 static int reserve_common(Workspace &w, int npairs) {
     int rc;
     int mt = max_tasks_for(npairs);
@@ -269,7 +277,7 @@ static int reserve_common(Workspace &w, int npairs) {
     if ((rc = w.bins.ensure((size_t)4 * kNumBins * 4))) return rc;
     if ((rc = w.segs.ensure((size_t)2 * 3 * (2 * kRBins + 1) * 4))) return rc;
     if ((rc = w.counters.ensure(256 * 4))) return rc;
-    if ((rc = w.totals.ensure(8 * 8))) return rc;
+    if ((rc = w.totals.ensure((size_t)(kTotSteps + 2 * 2 * kRBins) * 8))) return rc;
     return kOk;
 }
 
@@ -364,7 +372,8 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                     io.alen};
         t = TaskArgs{bins + (2 * pass) * kNumBins, bins + (2 * pass + 1) * kNumBins,
                      segs + (3 * pass) * NS, segs + (3 * pass + 1) * NS,
-                     segs + (3 * pass + 2) * NS, cnt + 4 * pass, cnt + 4 * pass + 1};
+                     segs + (3 * pass + 2) * NS, cnt + 4 * pass, cnt + 4 * pass + 1,
+                     reinterpret_cast<unsigned long long *>(tot + kTotSteps + pass * 2 * kRBins)};
     };
     KeyConsts kc16 = make_consts(sc, kOneHot), kc32 = make_consts(sc, kBytes);
     int pos = std::max(std::max(std::max(sc.match, sc.mismatch), std::max(sc.gap_open, sc.gap_ext)), 0);
@@ -383,11 +392,15 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                                  w.scan_sums.as<int64_t>(), mt, st));
     int64_t words1 = 0;
     int32_t seg1[2 * kRBins + 1], seg2[2 * kRBins + 1], cnt_h[16];
+    int64_t steps1[2 * kRBins], steps2[2 * kRBins];
     if (sync_sizes) {
         // trace size, per-class task counts (only non-empty classes are
-        // launched) and the error flag in one round trip
+        // launched), per-class sweep steps (their cost estimate) and the
+        // error flag in one round trip
         SWIFT_HIP_TRY(hipMemcpyAsync(&words1, tot + 0, 8, hipMemcpyDeviceToHost, st));
         SWIFT_HIP_TRY(hipMemcpyAsync(seg1, t1.seg_task0, sizeof(seg1), hipMemcpyDeviceToHost, st));
+        SWIFT_HIP_TRY(hipMemcpyAsync(steps1, tot + kTotSteps, sizeof(steps1),
+                                     hipMemcpyDeviceToHost, st));
         SWIFT_HIP_TRY(hipMemcpyAsync(cnt_h, cnt, sizeof(cnt_h), hipMemcpyDeviceToHost, st));
         SWIFT_HIP_TRY(hipStreamSynchronize(st));
         if (cnt_h[kCntError]) {
@@ -407,13 +420,18 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                  io.score, io.start, io.end, io.alen, io.qcons, io.rcons, io.cons_off,
                  tuning().walk_deferred};
     // One persistent kernel per (class, R) segment, each reading its task
-    // range from the device (empty segments exit at once), spread longest
-    // rows first over the side streams so the classes run concurrently and
-    // one class's tail overlaps the next class's work.
-    // seg_host (task counts read back) skips empty classes; without it every
-    // class kernel is launched and empty ones exit at once.
-    auto launch_classes = [&](const TaskArgs &t, bool with16, int pass,
-                              const int32_t *seg_host) -> int {
+    // range from the device (empty segments exit at once), spread over the
+    // side streams so the classes run concurrently and one class's tail
+    // overlaps the next class's work.
+    // With the host-known sizes (seg_host: task counts, steps_host: sweep
+    // steps per segment) empty classes are skipped and the classes are
+    // spread longest first onto the least loaded stream (LPT), cost = steps
+    // x instructions per step (linear in R), and each class's deferred walk
+    // follows its sweep on the same stream, overlapping the sweeps still
+    // running on the others.  Without sizes every class kernel is launched
+    // round-robin and the walk follows all sweeps.
+    auto launch_classes = [&](const TaskArgs &t, bool with16, int pass, const int32_t *seg_host,
+                              const int64_t *steps_host) -> int {
         int live[2 * kRBins], nlive = 0;
         for (int cls = with16 ? 0 : 1; cls < 2; ++cls) {
             for (int R = kRBins; R >= 1; --R) {
@@ -425,28 +443,43 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
         }
         if (nlive == 0) return kOk;
         const int nside = std::min(nlive, (int)Workspace::kSide);
+        int lane_of[2 * kRBins];
+        if (steps_host) {
+            double cost[2 * kRBins];
+            for (int li = 0; li < nlive; ++li) {
+                const int seg = live[li], R = kRBins - seg % kRBins;
+                cost[seg] = (double)steps_host[seg] * (26.0 + 20.0 * R);
+            }
+            std::stable_sort(live, live + nlive, [&](int x, int y) { return cost[x] > cost[y]; });
+            double load[Workspace::kSide] = {};
+            for (int li = 0; li < nlive; ++li) {
+                const int seg = live[li];
+                int best = 0;
+                for (int i = 1; i < nside; ++i)
+                    if (load[i] < load[best]) best = i;
+                load[best] += cost[seg];
+                lane_of[seg] = best;
+            }
+        } else {
+            for (int li = 0; li < nlive; ++li) lane_of[live[li]] = li % nside;
+        }
         SWIFT_HIP_TRY(hipEventRecord(w.fork, st));
         for (int i = 0; i < nside; ++i)
             SWIFT_HIP_TRY(hipStreamWaitEvent(w.side[i], w.fork, 0));
-        int k = 0;
         for (int li = 0; li < nlive; ++li) {
-            {
-                const int seg = live[li], cls = seg / kRBins, R = kRBins - seg % kRBins;
-                BatchArgs b = ba;
-                b.task_lo = t.seg_task0 + seg;
-                b.task_hi = t.seg_task0 + seg + 1;
-                b.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
-                b.kc = cls == 0 ? kc16 : kc32;
-                hipStream_t s2 = w.side[k++ % nside];
-                SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, tuning().waves_per_cu, s2));
-                if (ba.walk_deferred && seg_host) {
-                    // the class's walk follows its sweep on the same stream and
-                    // overlaps the sweeps of the classes still running
-                    const int slots = cls == 0 ? kSlots16 : kSlots32;
-                    const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;
-                    SWIFT_HIP_TRY(launch_walk(b, b.order, t.seg_first + seg, t.seg_len + seg, maxp,
-                                              w.moves.as<uint32_t>(), kMoveWords, s2));
-                }
+            const int seg = live[li], cls = seg / kRBins, R = kRBins - seg % kRBins;
+            BatchArgs b = ba;
+            b.task_lo = t.seg_task0 + seg;
+            b.task_hi = t.seg_task0 + seg + 1;
+            b.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
+            b.kc = cls == 0 ? kc16 : kc32;
+            hipStream_t s2 = w.side[lane_of[seg]];
+            SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, tuning().waves_per_cu, s2));
+            if (ba.walk_deferred && seg_host) {
+                const int slots = cls == 0 ? kSlots16 : kSlots32;
+                const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;
+                SWIFT_HIP_TRY(launch_walk(b, b.order, t.seg_first + seg, t.seg_len + seg, maxp,
+                                          w.moves.as<uint32_t>(), kMoveWords, s2));
             }
         }
         for (int i = 0; i < nside; ++i) {
@@ -456,7 +489,9 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
         return kOk;
     };
     SWIFT_HIP_TRY(mark(1, st, w.timed));
-    if ((rc = launch_classes(t1, !force32, 0, sync_sizes ? seg1 : nullptr))) return rc;
+    if ((rc = launch_classes(t1, !force32, 0, sync_sizes ? seg1 : nullptr,
+                             sync_sizes ? steps1 : nullptr)))
+        return rc;
 
     SWIFT_HIP_TRY(mark(2, st, w.timed));
     // ---- pass 2: int32 re-run of pairs flagged by the int16 kernel
@@ -473,6 +508,8 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             SWIFT_HIP_TRY(hipMemcpyAsync(&words2, tot + 1, 8, hipMemcpyDeviceToHost, st));
             SWIFT_HIP_TRY(hipMemcpyAsync(&nre, cnt + 5, 4, hipMemcpyDeviceToHost, st));
             SWIFT_HIP_TRY(hipMemcpyAsync(seg2, t2.seg_task0, sizeof(seg2), hipMemcpyDeviceToHost, st));
+            SWIFT_HIP_TRY(hipMemcpyAsync(steps2, tot + kTotSteps + 2 * kRBins, sizeof(steps2),
+                                         hipMemcpyDeviceToHost, st));
             SWIFT_HIP_TRY(hipStreamSynchronize(st));
             w.stats.ntasks32 += seg2[2 * kRBins] - seg2[kRBins];
             w.stats.nrerun = nre;
@@ -492,7 +529,9 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
         }
         if (!sync_sizes || nre > 0) {
             ba.order = w.order2.as<int32_t>();
-            if ((rc = launch_classes(t2, false, 1, sync_sizes ? seg2 : nullptr))) return rc;
+            if ((rc = launch_classes(t2, false, 1, sync_sizes ? seg2 : nullptr,
+                                     sync_sizes ? steps2 : nullptr)))
+                return rc;
         }
     }
     w.stats.trace_words = words2;

@@ -105,67 +105,87 @@ __device__ __forceinline__ uint32_t byte_rcode(unsigned char c) {
     return (uint32_t)u | (u == 'N' ? 0u : 0xFF00u);
 }
 
-// substitution flag (0/1 per element); one: 1 per element, a run-time value
-template <bool PK, int POL>
-__device__ __forceinline__ uint32_t sub_flag(uint32_t q, uint32_t r, uint32_t one) {
-    if constexpr (POL == kOneHot) {
-        // q: per element class mask of the query byte, r: class index of the
-        // reference byte; 1 = match
-        if constexpr (PK) return as_u(as_us(q) >> as_us(r)) & one;
-        else return (q >> r) & one;
-    } else {
-        static_assert(!PK, "byte policy is int32 only");
-        uint32_t x = (q ^ r) & ((q & r) >> 8) & 0xFFu;
-        return min(x, 1u);        // 1 = mismatch
-    }
+// Byte policy (int32 lanes): 1 = mismatch.
+__device__ __forceinline__ uint32_t byte_mismatch(uint32_t q, uint32_t r) {
+    uint32_t x = (q ^ r) & ((q & r) >> 8) & 0xFFu;
+    return min(x, 1u);
+}
+
+// Packed one-hot policy: the match flag of both halves from one v_perm_b32.
+// A row holds the query's match table over the reference classes A C G T
+// (0-3): 4 bytes per half, 0xFF where the query byte matches the class
+// (query N: all four; any other query byte: none).  A column's word holds,
+// per half, a perm selector in both bytes: class k -> k (half 0, table in
+// bytes 0-3) or 4 + k (half 1, bytes 4-7); reference N -> 13 (0xFF, it
+// matches every query byte); a padding column or another byte -> 12 (0x00).
+// perm(table half 1, table half 0, selector) is 0xFFFF per matching half.
+__host__ __device__ constexpr uint32_t query_table(uint32_t mask) {
+    uint32_t t = 0;
+    for (int k = 0; k < 4; ++k)
+        if ((mask >> k) & 1u) t |= 0xFFu << (8 * k);
+    return t;
+}
+__host__ __device__ constexpr uint32_t column_selector(uint32_t cls, int half) {
+    const uint32_t s = cls < 4 ? cls + 4u * (uint32_t)half : cls == 4 ? 13u : 12u;
+    return s * 0x0101u;
 }
 
 // ---------------------------------------------------------------------------
 // The staggered sweep of one lane group.  G lanes, R rows per lane.
 // Each lane slot carries PK ? 2 : 1 pairs.
 // ---------------------------------------------------------------------------
+//
+// The packed kernels (PK) run every step unmasked: they take only scorings
+// whose mismatch and gap moves all lose score (mismatch < 0, gaps <= -1), so
+// cells left of column 1 stay in the column-0 state and cells right of a
+// pair's last column (padding selectors: never a match) score strictly below
+// the pair's best real cell.  The left exit is kept >= 0 by a saturating
+// subtract, which makes the zero candidate of the max implicit.
 template <int G, int R, bool PK, int POL>
 struct Sweep {
+    static_assert(PK == (POL == kOneHot), "packed lanes use the one-hot policy, int32 the bytes");
     static constexpr int KPW = PK ? 8 : 16;          // rows per trace word
     static constexpr int W = (R + KPW - 1) / KPW;     // trace words per step
     static constexpr uint32_t H4MASK = PK ? 0xFFFCFFFCu : ~3u;
     static constexpr uint32_t CMASK = PK ? 0x00030003u : 3u;
     static constexpr int CTRL = G == 16 ? kRowShr1 : kWaveShr1;
 
-    uint32_t q[R];        // query codes of this lane's rows
+    uint32_t q[R];             // int32: query byte codes; PK: match tables of half 0
+    uint32_t q2[PK ? R : 1];   // PK: match tables of half 1
     uint32_t dr[R];       // DR of (row, j-1)
     uint32_t el[R];       // EL of (row, j-1)
     uint32_t best[R];     // int32: H<<12 | (4095-j) ; PK: row max of 4H
     uint32_t bcol[R];     // PK only: first column of the row max (both halves)
     uint32_t eu_last, dr_last, dr_top;
 
-    uint32_t c_dr, c_dsub, c_eo2, c_eo1, c_de, c_one, c_sign;
-    // PK: byte tables of the gap exits indexed by direction code, laid out
-    // for v_perm_b32 (entry of code c at byte 2c+1, sign via selector 8+c)
-    uint32_t lut_u_lo, lut_u_hi, lut_l_lo, lut_l_hi;
+    uint32_t c_dr, c_dsub, c_eo2, c_eo1, c_de, c_sign;
+    // PK: 16-bit tables indexed by direction code, laid out for v_perm_b32
+    // (entry of code c in bytes 2c, 2c+1): the up exit, and the left exit's
+    // penalty (subtracted with saturation at 0)
+    uint32_t lut_u_lo, lut_u_hi, pen_l_lo, pen_l_hi;
 
     static __device__ __forceinline__ void lut(int v0, int v1, int v2, int v3, uint32_t &lo,
                                                uint32_t &hi) {
-        lo = ((uint32_t)(v0 & 0xFF) << 8) | ((uint32_t)(v1 & 0xFF) << 24);
-        hi = ((uint32_t)(v2 & 0xFF) << 8) | ((uint32_t)(v3 & 0xFF) << 24);
+        lo = (uint32_t)(v0 & 0xFFFF) | ((uint32_t)(v1 & 0xFFFF) << 16);
+        hi = (uint32_t)(v2 & 0xFFFF) | ((uint32_t)(v3 & 0xFFFF) << 16);
     }
 
     __device__ __forceinline__ void init(const KeyConsts &k) {
         c_dr = splat<PK>(k.dr);
-        c_dsub = splat<PK>(k.dsub);
+        c_dsub = splat<PK>(PK ? -k.dsub : k.dsub);  // PK: times a 0xFFFF flag
         c_eo2 = splat<PK>(k.eo2);
         c_eo1 = splat<PK>(k.eo1);
         c_de = splat<PK>(k.de);
-        c_one = splat<PK>(k.one);
         c_sign = splat<PK>(k.sign);
         // up exit: open unless the cell came from above (code 2); left exit:
         // open unless it came from the left (code 1)
         lut(k.eo2, k.eo2, k.eo2 + k.de, k.eo2, lut_u_lo, lut_u_hi);
-        lut(k.eo1, k.eo1 + k.de, k.eo1, k.eo1, lut_l_lo, lut_l_hi);
+        lut(-k.eo1, -(k.eo1 + k.de), -k.eo1, -k.eo1, pen_l_lo, pen_l_hi);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             dr[r] = c_dr;   // column 0: H = 0, start
-            el[r] = c_eo1;  // column 0 is not a left move -> open
+            // column 0 is not a left move -> open; PK: clamped at 0
+            el[r] = PK ? 0u : c_eo1;
             best[r] = 0;
             bcol[r] = 0;
         }
@@ -198,9 +218,14 @@ struct Sweep {
         uint32_t pre[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            uint32_t f = sub_flag<PK, POL>(q[r], rr, c_one);
-            uint32_t dk = vmad<PK>(f, c_dsub, r == 0 ? diag0 : dr[r - 1]);
-            pre[r] = vmax<PK>(vmax<PK>(dk, el[r]), 0u);
+            const uint32_t diag = r == 0 ? diag0 : dr[r - 1];
+            if constexpr (PK) {
+                const uint32_t f = __builtin_amdgcn_perm(q2[r], q[r], rr);  // 0xFFFF: match
+                pre[r] = vmax<true>(vmad<true>(f, c_dsub, diag), el[r]);    // el >= 0
+            } else {
+                const uint32_t dk = vmad<false>(byte_mismatch(q[r], rr), c_dsub, diag);
+                pre[r] = vmax<false>(vmax<false>(dk, el[r]), 0u);
+            }
         }
         uint32_t eu = eu_in;
 #pragma unroll
@@ -211,20 +236,21 @@ struct Sweep {
             acc[r / KPW] |= c << (2 * (r % KPW));
             dr[r] = vadd<PK>(h4, c_dr);
             if constexpr (PK) {
-                // sign-extended per-half gap exit from the byte tables
-                uint32_t sel = c * 0x0102u + 0x08010801u;
+                // per-half gap exits from the 16-bit tables
+                const uint32_t sel = c * 0x0202u + 0x01000100u;
                 eu = vadd<true>(h4, __builtin_amdgcn_perm(lut_u_hi, lut_u_lo, sel));
-                el[r] = vadd<true>(h4, __builtin_amdgcn_perm(lut_l_hi, lut_l_lo, sel));
+                el[r] = as_u(__builtin_elementwise_sub_sat(
+                    as_us(h4), as_us(__builtin_amdgcn_perm(pen_l_hi, pen_l_lo, sel))));
             } else {
                 eu = vmad<PK>(opaque(vbit<PK>(4u, c)), c_de, vadd<PK>(h4, c_eo2));
                 el[r] = vmad<PK>(opaque(vbit<PK>(2u, c)), c_de, vadd<PK>(h4, c_eo1));
             }
             if constexpr (PK) {
-                uint32_t hv = MASKED ? (h4 & vmask) : h4;
-                uint32_t d = vsub<true>(best[r], hv);                  // < 0 <=> new strict max
-                uint32_t m = as_u(as_s(d) >> as_s(c_sign));  // 0xFFFF per new max
+                static_assert(!(PK && MASKED), "packed steps are never masked");
+                uint32_t d = vsub<true>(best[r], h4);              // < 0 <=> new strict max
+                uint32_t m = as_u(as_s(d) >> as_s(c_sign));        // 0xFFFF per new max
                 bcol[r] = (m & jj) | (~m & bcol[r]);
-                best[r] = vmax<true>(best[r], hv);
+                best[r] = vmax<true>(best[r], h4);
             } else {
                 uint32_t key = (h4 << (kColBits - 2)) | colterm;
                 if (MASKED) key &= vmask;
@@ -510,43 +536,48 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
         qo[h] = pid[h] >= 0 ? a.qoff[pid[h]] : 0;
         ro[h] = pid[h] >= 0 ? a.roff[pid[h]] : 0;
     }
-    // reference codes of the group's pairs into LDS (both halves packed)
+    // reference codes of the group's pairs into LDS (both halves packed),
+    // padding up to the last column any lane reaches (ncols + kBatchG - 2)
+    // and the step-ahead prefetch of it
     const int ncols = tk.steps - (kBatchG - 1);
     uint32_t exotic = 0;  // bit h: the pair's reference holds a byte outside ACGTNacgtn
-    for (int j = g; j < ncols; j += kBatchG) {
+    for (int j = g; j < ncols + kBatchG; j += kBatchG) {
         uint32_t v = 0;
 #pragma unroll
         for (int h = 0; h < PPG; ++h) {
-            uint32_t code = POL == kOneHot ? kRefOther : 0u;
+            uint32_t code = PK ? kRefPad : 0u;
             if (j < rl[h]) {
                 unsigned char c = (unsigned char)a.rseq[ro[h] + j];
-                code = POL == kOneHot ? ref_index_bf(c) : byte_rcode(c);
-                if (POL == kOneHot && code == kRefOther) exotic |= 1u << h;
+                code = PK ? ref_index_bf(c) : byte_rcode(c);
+                if (PK && code == kRefOther) exotic |= 1u << h;
             }
+            if constexpr (PK) code = column_selector(code, h);
             v |= code << (16 * h);
         }
         rr_lds[kPad + j] = v;
     }
-    if constexpr (POL == kOneHot) {
+    if constexpr (PK) {
 #pragma unroll
         for (int s2 = kBatchG / 2; s2 >= 1; s2 >>= 1) exotic |= __shfl_xor(exotic, s2, kBatchG);
     }
-    if (g < kPad) rr_lds[g] = POL == kOneHot ? (PK ? 0x00050005u : kRefOther) : 0u;
+    if (g < kPad)
+        rr_lds[g] = PK ? column_selector(kRefPad, 0) | column_selector(kRefPad, 1) << 16 : 0u;
     S sw;
     sw.init(a.kc);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int row = g * R + r;
-        uint32_t v = 0;
+        uint32_t v[PPG];
 #pragma unroll
         for (int h = 0; h < PPG; ++h) {
+            v[h] = 0;
             if (row < ql[h]) {
                 unsigned char c = (unsigned char)a.qseq[qo[h] + row];
-                uint32_t code = POL == kOneHot ? query_mask_bf(c) : byte_qcode(c);
-                v |= code << (16 * h);
+                v[h] = PK ? query_table(query_mask_bf(c)) : byte_qcode(c);
             }
         }
-        sw.q[r] = v;
+        sw.q[r] = v[0];
+        if constexpr (PK) sw.q2[r] = v[PPG - 1];
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_s_waitcnt(0);  // LDS writes of this wave visible to its lanes
@@ -573,9 +604,15 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
 #pragma unroll
         for (int w = 0; w < S::W; ++w) tr[trace_word<S::W>(t, w, lane)] = a1[w];
     };
+    // int32: masked steps keep the cells outside a pair's columns out of its
+    // row maxima.  Packed: no step is masked (see Sweep).
     const int T = tk.steps;
-    const int lo = min(kBatchG - 1, T), hi = max(lo, tk.min_rlen);
-    sweep_steps(T, lo, hi, run);
+    if constexpr (PK) {
+        for (int t = 0; t < T; ++t) run(std::false_type{}, t);
+    } else {
+        const int lo = min(kBatchG - 1, T), hi = max(lo, tk.min_rlen);
+        sweep_steps(T, lo, hi, run);
+    }
 
     // per half: lane best over valid rows, then group max
     GroupResult gres[PPG];
@@ -919,6 +956,7 @@ __global__ void __launch_bounds__(1024) sw_bin_scan(TaskArgs a) {
     __syncthreads();
     // segment (class, R) lengths in parallel, then their task offsets
     __shared__ int32_t seg_tasks[2 * kRBins];
+    if (tid < 2 * kRBins) a.seg_steps[tid] = 0;  // summed by sw_build_tasks
     if (tid < 2 * kRBins) {
         const int seg = tid, cls = seg / kRBins;
         int32_t len = 0;
@@ -966,13 +1004,11 @@ __global__ void __launch_bounds__(256) sw_bin_scatter(const int32_t *bin_of, int
 }
 
 // task descriptors (trace sizes filled, bases scanned afterwards)
-__global__ void sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_first,
-                               const int32_t *seg_len, const int32_t *ntasks_p,
-                               const int32_t *order, const int32_t *rlen, Task *tasks,
-                               int64_t *task_words) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int ntasks = *ntasks_p;
-    if (t >= ntasks) return;
+__device__ __forceinline__ void build_task(int t, const int32_t *seg_task0,
+                                           const int32_t *seg_first, const int32_t *seg_len,
+                                           const int32_t *order, const int32_t *rlen,
+                                           Task *tasks, int64_t *task_words,
+                                           unsigned long long *steps_acc) {
     int seg = 0;
     while (seg + 1 < 2 * kRBins && seg_task0[seg + 1] <= t) ++seg;
     int cls = seg / kRBins;
@@ -998,6 +1034,25 @@ __global__ void sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_firs
     int kpw = cls == 0 ? 8 : 16;
     int W = (R + kpw - 1) / kpw;
     task_words[t] = (int64_t)((tk.steps + 3) & ~3) * W * 64;  // whole 4-step granules
+    atomicAdd(&steps_acc[seg], (unsigned long long)tk.steps);
+}
+
+__global__ void __launch_bounds__(256)
+sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_first, const int32_t *seg_len,
+               const int32_t *ntasks_p, const int32_t *order, const int32_t *rlen, Task *tasks,
+               int64_t *task_words, unsigned long long *seg_steps) {
+    // per-segment sums of the tasks' sweep steps (the host's cost estimate
+    // of each class kernel), reduced in LDS before one atomic per segment
+    __shared__ unsigned long long steps_acc[2 * kRBins];
+    if (threadIdx.x < 2 * kRBins) steps_acc[threadIdx.x] = 0;
+    __syncthreads();
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int ntasks = *ntasks_p;
+    if (t < ntasks) build_task(t, seg_task0, seg_first, seg_len, order, rlen, tasks, task_words,
+                               steps_acc);
+    __syncthreads();
+    if (threadIdx.x < 2 * kRBins && steps_acc[threadIdx.x])
+        atomicAdd(&seg_steps[threadIdx.x], steps_acc[threadIdx.x]);
 }
 
 struct StoreTaskBase {
@@ -1124,7 +1179,7 @@ hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order
     if (max_tasks > 0) {
         hipLaunchKernelGGL(sw_build_tasks, dim3((max_tasks + 255) / 256), dim3(256), 0, st,
                            ta.seg_task0, ta.seg_first, ta.seg_len, ta.ntasks, order, bin.rlen,
-                           tasks, task_words);
+                           tasks, task_words, ta.seg_steps);
     }
     return launch_scan<int64_t>(LoadArr<int64_t>{task_words}, StoreTaskBase{tasks}, ta.ntasks,
                                 max_tasks, scan_sums, base0, total_words, nullptr, st);

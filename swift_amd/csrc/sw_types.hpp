@@ -13,10 +13,9 @@ struct KeyConsts {
     int eo2;   // 4*gapOpen + 2 (up exit, open)
     int eo1;   // 4*gapOpen + 1 (left exit, open)
     int de;    // 4*(gapExt - gapOpen)
-    // 1 and 15, passed at run time: masks and shifts by them leave the
-    // optimiser no value range to turn 0/1 products and sign-mask merges of
-    // packed halves into per-half compares and selects
-    int one;
+    // 15, passed at run time: an arithmetic shift by it leaves the optimiser
+    // no sign-splat to turn a masked merge of packed halves into per-half
+    // compares and selects
     int sign;
 };
 
@@ -36,7 +35,6 @@ inline KeyConsts make_consts(const Scoring &s, int policy) {
     k.eo2 = 4 * s.gap_open + 2;
     k.eo1 = 4 * s.gap_open + 1;
     k.de = 4 * (s.gap_ext - s.gap_open);
-    k.one = 1;
     k.sign = 15;
     return k;
 }
@@ -122,6 +120,7 @@ struct TaskArgs {
     int32_t *seg_len;         // pairs in each segment
     int32_t *ntasks;
     int32_t *npairs_binned;
+    unsigned long long *seg_steps;  // sweep steps summed over each segment's tasks
 };
 
 // launch helpers (sw_kernels.hip)

@@ -85,6 +85,9 @@ __host__ __device__ inline uint32_t query_code(unsigned char c) {
 // query byte a bit mask over the classes it matches, so
 // match(q, r) <=> (query_mask(q) >> ref_index(r)) & 1.
 constexpr uint32_t kRefOther = 5;
+// Class of the padding columns around a pair's reference in the packed
+// kernels: no query mask has bit 6, so a padding column matches nothing.
+constexpr uint32_t kRefPad = 6;
 __host__ __device__ constexpr uint32_t ref_index(unsigned char c) {
     switch (c) {
     case 'A': case 'a': return 0;

@@ -154,6 +154,29 @@ def test_batch_scorings_vs_oracle(A, orc, sc):
         compare(res.record(i), orc.sw(rs[i], qs[i], *sc), (i, sc))
 
 
+@pytest.mark.parametrize("sc", [(2, -1, -10, -1), (1, -1, -1, -1), (1, 0, -1, 0)])
+def test_batch_padding_columns(A, orc, sc):
+    """Queries of N (which matches every reference base) over references a
+    few columns apart in length, so each task sweeps columns past the end of
+    most of its pairs: the padding columns must never score as a match, and
+    the best cell must stay inside the pair whether or not the steps are
+    masked (masking is skipped when mismatch and gaps all lose score)."""
+    rnd = random.Random(sum(sc) + 7)
+    qs, rs = [], []
+    for i in range(400):
+        ql = rnd.randint(1, 64)
+        rl = rnd.randint(1, 60)
+        if i % 3 == 0:
+            q = b"N" * ql
+        else:
+            q = bytes(rnd.choice(b"ACGTNNNN") for _ in range(ql))
+        qs.append(q)
+        rs.append(bytes(rnd.choice(b"ACGTN") for _ in range(rl)))
+    res = A.sw_batch(qs, rs, A.Scoring(*sc))
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i], *sc), (i, sc))
+
+
 def test_batch_edge_sizes(A, orc):
     qs = [b"A", b"ACGT" * 80, b"T" * 16, b"G" * 17, b"ACGTN", b"CCCCCCCCCC"]
     rs = [b"A", b"ACGT" * 120, b"T", b"G" * 480, b"N", b"GGGGGGGGGG"]
