@@ -475,9 +475,9 @@ def main():
     if tuned is not None:
         out["autotune"] = {"sample_pairs": n_tune,
                            "chosen": {x: tuned[x] for x in ("waves_per_cu", "allow_int16",
-                                                             "walk_deferred")},
+                                                             "walk_deferred", "coarse_rows")},
                            "candidates_ms": [[r["waves_per_cu"], r["int16"], r["walk_deferred"],
-                                              r["ms"]] for r in tuned["table"]]}
+                                              r["coarse_rows"], r["ms"]] for r in tuned["table"]]}
     if not args.no_plain:
         groups = plain_pass_setup(qbuf, qoff, qlen, rbuf, roff, rlen)
         plain_pass(groups)

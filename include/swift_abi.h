@@ -119,14 +119,25 @@ int swift_sw_batch_timing(int enable, float *ms4);
 int swift_set_tuning(int waves_per_cu, int allow_int16, int walk_deferred);
 int swift_get_tuning(int *waves_per_cu, int *allow_int16, int *walk_deferred);
 
+/* Row classes of a batched call: each pair runs in the kernel of its rows
+ * per lane R = ceil(query / 16).  1 = R rounded up to a multiple of 5 (at
+ * most 4 class kernels: lower latency for small batches, some padding rows);
+ * 0 = every R its own class; -1 (default) = 1 for batches under 131072 pairs,
+ * inside graph capture and on serving slots, else 0.  No counterpart in the
+ * reference; results are identical in every mode. */
+int swift_set_coarse_rows(int mode);
+int swift_get_coarse_rows(int *mode);
+
 /* Autotuner of the batched path: times every candidate setting (waves per
- * CU in {max, 8, 12, 16, 20, 24} x walk fused/deferred with int16 lanes, and
- * int32-only lanes) on the given pairs, reps timed runs each after a warm-up,
- * with HIP events on `stream`, then applies the fastest.  Outputs are
+ * CU in {max, 8, 12, 16, 20, 24} x walk fused/deferred with int16 lanes, the
+ * coarse row classes, and int32-only lanes) on the given pairs, reps timed
+ * runs each after a warm-up, with HIP events on `stream`, then applies the
+ * fastest (swift_get_coarse_rows reports its row-class mode).  Outputs are
  * written by the timed runs (identical under every setting).  table_ms
- * (optional, 4 floats per candidate: waves, int16, walk, ms) receives up to
- * table_cap rows; *ntable the number of candidates timed.  Changes the
- * process-wide tuning: call it while no other batched call is running. */
+ * (optional, 5 floats per candidate: waves, int16, walk, coarse rows, ms)
+ * receives up to table_cap rows; *ntable the number of candidates timed.
+ * Changes the process-wide tuning: call it while no other batched call is
+ * running. */
 int swift_autotune_batch(const char *qseq_d, const int64_t *qoff_d, const int32_t *qlen_d,
                          const char *rseq_d, const int64_t *roff_d, const int32_t *rlen_d,
                          int npairs, int maxq, int maxr, int match, int mismatch, int gap_open,

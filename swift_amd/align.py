@@ -148,7 +148,7 @@ class PairBatch:
     def autotune(self, scoring=Scoring(), reps=2) -> dict:
         """Times the batched path's settings on these pairs and applies the
         fastest (see swift_autotune_batch)."""
-        table = (C.c_float * (4 * 32))()
+        table = (C.c_float * (5 * 32))()
         bw, bi, bk, nt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         bms = C.c_float()
         check(lib().swift_autotune_batch(
@@ -159,11 +159,13 @@ class PairBatch:
             C.c_void_p(self.end.ptr), C.c_void_p(self.alen.ptr), C.c_void_p(self.qc.ptr),
             C.c_void_p(self.rc.ptr), C.c_void_p(self.co.ptr), reps, None, C.byref(bw),
             C.byref(bi), C.byref(bk), C.byref(bms), table, 32, C.byref(nt)))
-        rows = [dict(waves_per_cu=int(table[4 * i]), int16=bool(table[4 * i + 1]),
-                     walk_deferred=bool(table[4 * i + 2]), ms=round(float(table[4 * i + 3]), 3))
+        rows = [dict(waves_per_cu=int(table[5 * i]), int16=bool(table[5 * i + 1]),
+                     walk_deferred=bool(table[5 * i + 2]), coarse_rows=bool(table[5 * i + 3]),
+                     ms=round(float(table[5 * i + 4]), 3))
                 for i in range(nt.value)]
         return dict(waves_per_cu=bw.value, allow_int16=bool(bi.value),
-                    walk_deferred=bool(bk.value), ms=round(bms.value, 3), table=rows)
+                    walk_deferred=bool(bk.value), coarse_rows=get_coarse_rows(),
+                    ms=round(bms.value, 3), table=rows)
 
 
 def sw_batch(queries, refs, scoring=Scoring()) -> Alignments:
@@ -187,4 +189,17 @@ def set_tuning(waves_per_cu=0, allow_int16=True, walk_deferred=False):
 def get_tuning() -> dict:
     a, b, c = C.c_int(), C.c_int(), C.c_int()
     check(lib().swift_get_tuning(C.byref(a), C.byref(b), C.byref(c)))
-    return dict(waves_per_cu=a.value, allow_int16=bool(b.value), walk_deferred=bool(c.value))
+    return dict(waves_per_cu=a.value, allow_int16=bool(b.value), walk_deferred=bool(c.value),
+                coarse_rows=get_coarse_rows())
+
+
+def set_coarse_rows(mode: int) -> None:
+    """1: rows per lane rounded up to a multiple of 5 (at most 4 row-class
+    kernels); 0: every R its own class; -1: chosen per call by batch size."""
+    check(lib().swift_set_coarse_rows(int(mode)))
+
+
+def get_coarse_rows() -> int:
+    m = C.c_int()
+    check(lib().swift_get_coarse_rows(C.byref(m)))
+    return m.value

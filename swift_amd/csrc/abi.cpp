@@ -123,4 +123,18 @@ int swift_get_tuning(int *waves_per_cu, int *allow_int16, int *walk_deferred) {
     return kOk;
 }
 
+int swift_set_coarse_rows(int mode) {
+    if (mode < -1 || mode > 1) {
+        set_error("coarse row class mode must be -1, 0 or 1");
+        return kErrInvalid;
+    }
+    tuning().coarse_rows = mode;
+    return kOk;
+}
+
+int swift_get_coarse_rows(int *mode) {
+    if (mode) *mode = tuning().coarse_rows;
+    return kOk;
+}
+
 }  // extern "C"

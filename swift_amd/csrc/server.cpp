@@ -310,7 +310,10 @@ int swift_server_create(const swift_index *ix, int nslots, int64_t mem_budget_by
     }
     for (int i = 0; i < nslots; ++i) {
         auto *sl = new SrvSlot;
-        sl->sws = workspace_new();
+        // the slot's classes run on its own stream: with several slots, side
+        // streams would outnumber the hardware queues, and a class kernel of
+        // one slot would then hold up another slot's work queued behind it
+        sl->sws = workspace_new(0);
         hipError_t e = hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking);
         if (e != hipSuccess) {
             set_error("hipStreamCreate: %s", hipGetErrorString(e));

@@ -82,6 +82,7 @@ struct Workspace {
     // queues a process gets by default, so no two side streams share one
     // (kernels of streams on one hardware queue run one after another).
     static constexpr int kSide = 3;
+    int nside = kSide;        // side streams in use; 0: every class on the caller's stream
     hipStream_t side[kSide] = {};
     hipEvent_t fork = nullptr, join[kSide] = {};
     bool streams_ready = false;
@@ -118,9 +119,10 @@ static Workspace &ws() {
 }
 
 // private workspaces (one per serving slot)
-Workspace *workspace_new() {
+Workspace *workspace_new(int side_streams) {
     Workspace *w = new Workspace();
     init_workspace(w);
+    w->nside = std::max(0, std::min(side_streams, (int)Workspace::kSide));
     return w;
 }
 
@@ -132,7 +134,7 @@ void workspace_free(Workspace *w) {
         w->all_bufs[i]->bytes = 0;
     }
     if (w->streams_ready) {
-        for (int i = 0; i < Workspace::kSide; ++i) {
+        for (int i = 0; i < w->nside; ++i) {
             (void)hipStreamDestroy(w->side[i]);
             (void)hipEventDestroy(w->join[i]);
         }
@@ -151,7 +153,8 @@ BatchStats last_batch_stats() { return ws().stats; }
 
 static int ensure_streams(Workspace &w) {
     if (w.streams_ready) return kOk;
-    for (int i = 0; i < Workspace::kSide; ++i) {
+    // only the streams in use: an idle stream would still take a hardware queue
+    for (int i = 0; i < w.nside; ++i) {
         SWIFT_HIP_TRY(hipStreamCreateWithFlags(&w.side[i], hipStreamNonBlocking));
         SWIFT_HIP_TRY(hipEventCreateWithFlags(&w.join[i], hipEventDisableTiming));
     }
@@ -251,6 +254,8 @@ static int max_tasks_for(int npairs) { return (npairs + kSlots32 - 1) / kSlots32
 
 static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
     int R = std::max(1, (maxq + kBatchG - 1) / kBatchG);
+    // coarse row classes round R up (bound for either mode)
+    R = std::min(kRBins, (R + kCoarseRows - 1) / kCoarseRows * kCoarseRows);
     int64_t steps = (maxr + kBatchG - 1 + 3) & ~3;
     int64_t w16 = (R + 7) / 8, w32 = (r32_class(R) + 15) / 16;
     int64_t t16 = (npairs + kSlots16 - 1) / kSlots16 + 2 * kRBins;
@@ -366,10 +371,18 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     int32_t *segs = w.segs.as<int32_t>();
     const int NS = 2 * kRBins + 1;
 
+    // Coarse row classes for small batches, streams without side streams and
+    // graph capture: a class kernel whose tasks all fit one round of waves
+    // lasts about one task, so the batch's latency is the number of classes
+    // run one after another (or over the side streams) times a task; at most
+    // 4 classes instead of 20.  Large batches keep every R (no padding rows).
+    const int cmode = tuning().coarse_rows;
+    const int coarse =
+        cmode == 1 || (cmode < 0 && (w.nside == 0 || !sync_sizes || n < kCoarsePairs)) ? 1 : 0;
     auto pass_args = [&](int pass, const PairResult *prev, int f32, BinArgs &b, TaskArgs &t) {
         b = BinArgs{io.rseq, io.roff, io.qlen, io.rlen, prev, n, f32, w.bin_of.as<int32_t>(),
                     bins + (2 * pass) * kNumBins, cnt + kCntError, io.score, io.start, io.end,
-                    io.alen};
+                    io.alen, coarse};
         t = TaskArgs{bins + (2 * pass) * kNumBins, bins + (2 * pass + 1) * kNumBins,
                      segs + (3 * pass) * NS, segs + (3 * pass + 1) * NS,
                      segs + (3 * pass + 2) * NS, cnt + 4 * pass, cnt + 4 * pass + 1,
@@ -442,7 +455,8 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             }
         }
         if (nlive == 0) return kOk;
-        const int nside = std::min(nlive, (int)Workspace::kSide);
+        // nside 0: the classes run one after another on the caller's stream
+        const int nside = std::min(nlive, w.nside);
         int lane_of[2 * kRBins];
         if (steps_host) {
             double cost[2 * kRBins];
@@ -461,9 +475,9 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                 lane_of[seg] = best;
             }
         } else {
-            for (int li = 0; li < nlive; ++li) lane_of[live[li]] = li % nside;
+            for (int li = 0; li < nlive; ++li) lane_of[live[li]] = nside ? li % nside : 0;
         }
-        SWIFT_HIP_TRY(hipEventRecord(w.fork, st));
+        if (nside) SWIFT_HIP_TRY(hipEventRecord(w.fork, st));
         for (int i = 0; i < nside; ++i)
             SWIFT_HIP_TRY(hipStreamWaitEvent(w.side[i], w.fork, 0));
         for (int li = 0; li < nlive; ++li) {
@@ -473,9 +487,9 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             b.task_hi = t.seg_task0 + seg + 1;
             b.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
             b.kc = cls == 0 ? kc16 : kc32;
-            hipStream_t s2 = w.side[lane_of[seg]];
+            hipStream_t s2 = nside ? w.side[lane_of[seg]] : st;
             SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, tuning().waves_per_cu, s2));
-            if (ba.walk_deferred && seg_host) {
+            if (ba.walk_deferred && seg_host) {  // per-class walk
                 const int slots = cls == 0 ? kSlots16 : kSlots32;
                 const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;
                 SWIFT_HIP_TRY(launch_walk(b, b.order, t.seg_first + seg, t.seg_len + seg, maxp,

@@ -57,7 +57,9 @@ int reserve_batch(int npairs, int maxq, int maxr);
 // thread on its own stream.  sw_batch_run_in sizes the trace on the device
 // (it synchronises `st`).
 struct Workspace;
-Workspace *workspace_new();
+// side_streams: row-class kernels spread over that many private streams
+// (0..3); 0 runs them one after another on the caller's stream
+Workspace *workspace_new(int side_streams = 3);
 void workspace_free(Workspace *w);
 int64_t workspace_bytes(const Workspace *w);
 int sw_batch_run_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st);
@@ -88,7 +90,11 @@ struct Tuning {
     int int16 = 1;              // allow the int16 packed path
     int walk_deferred = 1;      // 1: walk-back kernels after each class's sweep (moves,
                                 // then strings); 0: the walk fused into each wave task
+    int coarse_rows = -1;       // 1: rows per lane rounded up to a multiple of 5 (at most
+                                // 4 row classes); 0: every R its own class; -1: per call
 };
+// batches below this many pairs take the coarse row classes under coarse_rows -1
+constexpr int kCoarsePairs = 131072;
 Tuning &tuning();
 
 }  // namespace swift

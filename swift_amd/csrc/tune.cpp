@@ -2,8 +2,9 @@
 // counterpart).  The knobs are the batched kernels' launch shape and lane
 // width: persistent waves per CU (occupancy of the row-class kernels), packed
 // int16 lanes on/off (two pairs per lane, int32 re-run of overflow suspects;
-// off = every pair in int32), and where the traceback walk runs (fused into
-// each wave task, or deferred to per-class walk kernels).  Results are
+// off = every pair in int32), where the traceback walk runs (fused into each
+// wave task, or deferred to walk kernels), and whether rows per lane are
+// rounded up to coarse row classes.  Results are
 // identical under every setting, so the tuner only times: each candidate
 // runs the caller's own pairs (or a prefix of them) `reps` times after a
 // warm-up, timed with HIP events on the caller's stream, and the fastest
@@ -17,7 +18,7 @@
 namespace swift {
 
 struct Candidate {
-    int waves_per_cu, int16, walk;
+    int waves_per_cu, int16, walk, coarse;
 };
 
 static std::vector<Candidate> candidates(int cu_waves_max) {
@@ -25,9 +26,13 @@ static std::vector<Candidate> candidates(int cu_waves_max) {
     const int waves[] = {0, 8, 12, 16, 20, 24};
     for (int walk = 0; walk < 2; ++walk)
         for (int w : waves)
-            if (w == 0 || w < cu_waves_max) c.push_back({w, 1, walk});
-    c.push_back({0, 0, 1});  // int32 lanes only
-    c.push_back({0, 0, 0});
+            if (w == 0 || w < cu_waves_max) c.push_back({w, 1, walk, 0});
+    for (int w : {0, 8}) {
+        c.push_back({w, 1, 1, 1});  // coarse row classes
+        c.push_back({w, 1, 0, 1});
+    }
+    c.push_back({0, 0, 1, 0});  // int32 lanes only
+    c.push_back({0, 0, 0, 0});
     return c;
 }
 
@@ -59,12 +64,13 @@ int swift_autotune_batch(const char *qseq_d, const int64_t *qoff_d, const int32_
     SWIFT_HIP_TRY(hipEventCreate(&e1));
     const auto cand = candidates(32);
     float best = FLT_MAX;
-    Candidate pick{saved.waves_per_cu, saved.int16, saved.walk_deferred};
+    Candidate pick{saved.waves_per_cu, saved.int16, saved.walk_deferred, saved.coarse_rows};
     int k = 0, rc = kOk;
     for (const Candidate &c : cand) {
         tuning().waves_per_cu = c.waves_per_cu;
         tuning().int16 = c.int16;
         tuning().walk_deferred = c.walk;
+        tuning().coarse_rows = c.coarse;
         if ((rc = sw_batch_run(io, sc, st, true))) break;  // warm-up (sizes buffers)
         float ms = 0;
         for (int r = 0; r < reps && rc == kOk; ++r) {
@@ -79,10 +85,11 @@ int swift_autotune_batch(const char *qseq_d, const int64_t *qoff_d, const int32_
         if (rc) break;
         ms /= reps;
         if (table_ms && k < table_cap) {
-            table_ms[4 * k + 0] = (float)c.waves_per_cu;
-            table_ms[4 * k + 1] = (float)c.int16;
-            table_ms[4 * k + 2] = (float)c.walk;
-            table_ms[4 * k + 3] = ms;
+            table_ms[5 * k + 0] = (float)c.waves_per_cu;
+            table_ms[5 * k + 1] = (float)c.int16;
+            table_ms[5 * k + 2] = (float)c.walk;
+            table_ms[5 * k + 3] = (float)c.coarse;
+            table_ms[5 * k + 4] = ms;
         }
         ++k;
         if (ms < best) {
@@ -100,6 +107,7 @@ int swift_autotune_batch(const char *qseq_d, const int64_t *qoff_d, const int32_
     tuning().waves_per_cu = pick.waves_per_cu;
     tuning().int16 = pick.int16;
     tuning().walk_deferred = pick.walk;
+    tuning().coarse_rows = pick.coarse;
     if (best_waves) *best_waves = pick.waves_per_cu;
     if (best_int16) *best_int16 = pick.int16;
     if (best_walk) *best_walk = pick.walk;

@@ -202,14 +202,16 @@ def test_batch_repeatable(A):
     assert np.array_equal(r1.score, r2.score) and r1.qry_cons == r2.qry_cons
 
 
-@pytest.mark.parametrize("deferred", [False, True])
-def test_batch_walk_modes(A, orc, deferred):
-    """Fused and deferred walk-back give identical results (C3 shapes, an
-    int16-overflow re-run and exotic bytes in the same call)."""
+@pytest.mark.parametrize("deferred,coarse", [(False, 0), (True, 0), (False, 1), (True, 1)])
+def test_batch_walk_modes(A, orc, deferred, coarse):
+    """Fused and deferred walk-back, exact or coarse row classes, all give the
+    oracle's results (C3 shapes, an int16-overflow re-run and exotic bytes in
+    the same call)."""
     from swift_amd import synth
     before = A.get_tuning()
     try:
         A.set_tuning(before["waves_per_cu"], True, deferred)
+        A.set_coarse_rows(coarse)
         qs, rs = synth.c3_pairs(1500, seed=17)
         rnd = random.Random(17)
         q2, r2 = rand_pairs(rnd, 200, 120, 300, [b"ACGTRY*", b"ACGTacgtNn#"])
@@ -228,6 +230,7 @@ def test_batch_walk_modes(A, orc, deferred):
             compare(res.record(i), orc.sw(reg[i], qo[i], 200, -1, -10, -1), i)
     finally:
         A.set_tuning(before["waves_per_cu"], before["allow_int16"], before["walk_deferred"])
+        A.set_coarse_rows(before["coarse_rows"])
 
 
 def test_autotune_keeps_results(A, orc):
@@ -241,6 +244,8 @@ def test_autotune_keeps_results(A, orc):
         t = b.autotune(reps=1)
         assert len(t["table"]) >= 10 and t["ms"] > 0
         assert A.get_tuning()["walk_deferred"] == t["walk_deferred"]
+        assert A.get_coarse_rows() == t["coarse_rows"]
+        assert any(r["coarse_rows"] for r in t["table"])
         b.run()
         res = b.results()
         sc, st, en, al, qc, rc = orc.sw_batch(qs, rs, nthreads=8)
@@ -248,3 +253,4 @@ def test_autotune_keeps_results(A, orc):
         assert res.qry_cons == qc and res.ref_cons == rc
     finally:
         A.set_tuning(before["waves_per_cu"], before["allow_int16"], before["walk_deferred"])
+        A.set_coarse_rows(before["coarse_rows"])
