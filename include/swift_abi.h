@@ -305,6 +305,33 @@ int swift_records_cigar(const swift_records *rec, int max_records, char *cigar_d
 int swift_write_records(const char *path, int format, int only_kept, const swift_records *rec,
                         int nrec, const char *const *qry_names, const swift_index *ix);
 
+/* ======================================================== phase-2 drivers
+ * Whole-phase drop-ins of the reference's phase-2 drivers: read the phase-1
+ * match file (name\tstrand\trefIdx\tshift\trefPos\tnameOffset\tseqOffset),
+ * fetch each query and its reference window (refPos - 100, up to 300 bases)
+ * from the FASTA files by file offset, align every pair on the device and
+ * write printOutput text (outFormat 0) or SAM (outFormat 1).  Scores must be
+ * integral.  On error they print and exit(1), as the reference does on its
+ * I/O errors. */
+/* alignQueries3 (src/align.h:33-36): numMatches records; for text output
+ * the best record of each read is kept (chooseBestAlignment; ties: the
+ * first). */
+void alignQueries3(char *queryFileName, int maxQryLen, char *refFileName, char *matchFile,
+                   int outFormat, float match, float mismatch, float gapOpenPenalty,
+                   float gapExtPenalty, const char *outputFile, int numMatches);
+/* alignQueries_cpu_mt (src/align.h:54-56): every record, in the reference's
+ * rounds of 8 match lines (the last round repeats the last match when the
+ * line count is not a multiple of 8, as the reference's fgets loop does). */
+void alignQueries_cpu_mt(char *qryFile, int numQrs, int qryLen, char *refFile, char *matchFile,
+                         int outFormat, float match, float mismatch, float gapOpenPenalty,
+                         float gapExtPenalty, const char *outputFile);
+/* Status-returning form of both: gpuDriver 1 = alignQueries3's rules
+ * (numMatches records), 0 = alignQueries_cpu_mt's (numMatches ignored). */
+int swift_align_queries(const char *qryFile, int qryLen, const char *refFile,
+                        const char *matchFile, int outFormat, float match, float mismatch,
+                        float gapOpenPenalty, float gapExtPenalty, const char *outputFile,
+                        int numMatches, int gpuDriver);
+
 /* ======================================================== serving loop
  * No reference counterpart (the reference is a batch command-line tool).
  * A server keeps an index resident and aligns read batches submitted from

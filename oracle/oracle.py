@@ -94,6 +94,7 @@ def ref():
         _ref.ref_region.restype = C.c_int
         _ref.ref_format_record.restype = C.c_int
         _ref.ref_format_cigar.restype = C.c_int
+        _ref.ref_format_sam.restype = C.c_int
     return _ref
 
 

@@ -36,10 +36,15 @@ void refMapCreate(const char *file);
 void refMapFree();
 void refMapGetOffsets(int index, uint pos, uint *nameOffset, uint *seqOffset);
 void getRefSeq2(FILE *filePtr, ulong fileOffset, char *seq, int maxLength);
+void getSeq(FILE *filePtr, ulong fileOffset, char *seq, int maxLength);
+void getSeqName(FILE *filePtr, ulong fileOffset, char *name);
 void printOutput(int numQueries, char *queryName, char *refName, float *score,
                  int *refDistance, int *alignStart, int *alignEnd,
                  int *alignLength, int combinedLength, char *queryConsensus,
                  char *refConsensus, FILE *outputFilePtr);
+void printOutputSam(int numQueries, char *queryName, char *refName, float *score,
+        int *alignStart, int *alignEnd, int *alignLength, int combinedLength,
+        char *queryConsensus, char *refConsensus, FILE *outputFilePtr);
 void printCigar(char *queryConsensus, char *refConsensus, int alignLength,
                 FILE *outputFilePtr);
 
@@ -194,6 +199,42 @@ int ref_region(const char *fastaPath, int refIdx, int refPos, char *out,
     return (int) strlen(out);
 }
 
+/* getSeq / getSeqName on a file by path (the phase-2 drivers' query and
+ * name fetch, src/input.cu:37-52,163-204). */
+int ref_get_seq(const char *path, unsigned long offset, char *out, int maxLength)
+{
+    FILE *f = fopen(path, "r");
+    if (!f)
+        return -1;
+    out[0] = '\0';
+    getSeq(f, offset, out, maxLength);
+    fclose(f);
+    return (int) strlen(out);
+}
+
+int ref_get_seq_name(const char *path, unsigned long offset, char *out)
+{
+    FILE *f = fopen(path, "r");
+    if (!f)
+        return -1;
+    getSeqName(f, offset, out);
+    fclose(f);
+    return (int) strlen(out);
+}
+
+/* refMapCreate + refMapGetOffsets (src/refMap.cu:59-155). */
+int ref_map_offsets(const char *fastaPath, int refIdx, unsigned int pos,
+                    unsigned int *nameOffset, unsigned int *seqOffset)
+{
+    uint n = 0, sOff = 0;
+    refMapCreate(fastaPath);
+    refMapGetOffsets(refIdx, pos, &n, &sOff);
+    refMapFree();
+    *nameOffset = n;
+    *seqOffset = sOff;
+    return 0;
+}
+
 /* printOutput of one record into a memory buffer; returns its length. */
 int ref_format_record(const char *qryName, const char *refName, float score,
                       int refDistance, int start, int end, int alen,
@@ -212,6 +253,34 @@ int ref_format_record(const char *qryName, const char *refName, float score,
     FILE *f = tmpfile();
     printOutput(1, qn, rn, &score, &refDistance, &start, &end, &alen,
                 combined, qc, rc, f);
+    long n = ftell(f);
+    rewind(f);
+    if (n >= outCap)
+        n = outCap - 1;
+    size_t got = fread(out, 1, (size_t) n, f);
+    out[got] = '\0';
+    fclose(f);
+    free(qc);
+    free(rc);
+    return (int) got;
+}
+
+/* printOutputSam of one record into a memory buffer; returns its length. */
+int ref_format_sam(const char *qryName, const char *refName, float score, int start,
+                   int end, int alen, const char *qryCons, const char *refCons,
+                   int combined, char *out, int outCap)
+{
+    char qn[64], rn[64];
+    memset(qn, 0, sizeof(qn));
+    memset(rn, 0, sizeof(rn));
+    strncpy(qn, qryName, 49);
+    strncpy(rn, refName, 49);
+    char *qc = (char *) calloc((size_t) combined, 1);
+    char *rc = (char *) calloc((size_t) combined, 1);
+    memcpy(qc, qryCons, (size_t) alen);
+    memcpy(rc, refCons, (size_t) alen);
+    FILE *f = tmpfile();
+    printOutputSam(1, qn, rn, &score, &start, &end, &alen, combined, qc, rc, f);
     long n = ftell(f);
     rewind(f);
     if (n >= outCap)

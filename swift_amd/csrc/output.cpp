@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "output.hpp"
 #include "phase1_host.hpp"
 
 namespace swift {

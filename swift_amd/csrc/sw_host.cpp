@@ -198,6 +198,16 @@ static int to_int_score(float f, int *out, const char *what) {
     return kOk;
 }
 
+int scoring_from_floats(float match, float mismatch, float gap_open, float gap_ext, Scoring *sc) {
+    int rc;
+    if ((rc = to_int_score(match, &sc->match, "match")) ||
+        (rc = to_int_score(mismatch, &sc->mismatch, "mismatch")) ||
+        (rc = to_int_score(gap_open, &sc->gap_open, "gapOpenPenalty")) ||
+        (rc = to_int_score(gap_ext, &sc->gap_ext, "gapExtPenalty")))
+        return rc;
+    return kOk;
+}
+
 // ---------------------------------------------------------------------------
 // Plain path (alignSequences drop-in).
 // ---------------------------------------------------------------------------

@@ -49,6 +49,8 @@ struct BatchIO {
 };
 
 int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32);
+// float scores of the reference's interfaces -> integers (each must be integral)
+int scoring_from_floats(float match, float mismatch, float gap_open, float gap_ext, Scoring *sc);
 int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync_sizes);
 int reserve_batch(int npairs, int maxq, int maxr);
 

@@ -349,7 +349,24 @@ def search_queries3(qry_path, ref_path, match_path, k=12, max_hits=3, thr=1000,
     return nm.value
 
 
+def align_queries(qry_path, ref_path, match_path, out_path, qry_len, out_format=0,
+                  scoring=Scoring(), num_matches=None, gpu_driver=True) -> None:
+    """Phase 2 from the match file, as the reference's drivers run it:
+    gpu_driver=True follows alignQueries3 (src/align.cu:574-808: the first
+    num_matches records, text output keeps each read's best record), False
+    alignQueries_cpu_mt (src/align.cu:2363-2511: every record, in rounds of 8
+    match lines).  out_format 0 = printOutput text, 1 = SAM."""
+    if num_matches is None:
+        with open(match_path, "rb") as f:
+            num_matches = sum(1 for _ in f)
+    check(lib().swift_align_queries(
+        str(qry_path).encode(), int(qry_len), str(ref_path).encode(),
+        str(match_path).encode(), int(out_format), C.c_float(scoring.match),
+        C.c_float(scoring.mismatch), C.c_float(scoring.gap_open), C.c_float(scoring.gap_ext),
+        str(out_path).encode(), int(num_matches), 1 if gpu_driver else 0))
+
+
 __all__ = ["Index", "ReadBatch", "StrandHits", "RecordBuffers", "Records", "PipelineGraph",
            "Query", "get_hash",
            "revcomp", "lookup_table5_create", "lookup_table5_map_qry", "lookup_table5_delete",
-           "map_hits6_best_hits", "search_queries3", "SwiftError"]
+           "map_hits6_best_hits", "search_queries3", "align_queries", "SwiftError"]
