@@ -388,6 +388,64 @@ __device__ int gather_hits(const IndexDev &ix, const unsigned char *qb, int nq, 
     return total;
 }
 
+// The hits gather_hits lays out, for strands of at most kFlatRounds * 64
+// k-mers: every bucket lookup is issued first and every entry load of the
+// strand in one pass, several in flight per lane, so a strand costs two
+// dependent memory round trips instead of two per 64 k-mers.  ex / stt hold
+// kFlatRounds * 64 entries.
+constexpr int kFlatRounds = 4;
+__device__ int gather_hits_flat(const IndexDev &ix, const unsigned char *qb, int nq,
+                                uint64_t *kbuf, int64_t cap, int32_t *ex, int32_t *stt, bool *ok) {
+    const int lane = threadIdx.x & 63;
+    int cnt[kFlatRounds], st[kFlatRounds];
+#pragma unroll
+    for (int r = 0; r < kFlatRounds; ++r) {
+        const int i = r * 64 + lane;
+        cnt[r] = 0;
+        st[r] = 0;
+        if (i < nq) {
+            const int2 bk = ix.bucket[kmer_hash(qb + i, ix.k)];
+            st[r] = bk.x;
+            cnt[r] = bk.y;
+        }
+    }
+    int total = 0;
+#pragma unroll
+    for (int r = 0; r < kFlatRounds; ++r) {
+        const int incl = wave_incl_scan(cnt[r]);
+        ex[r * 64 + lane] = total + incl - cnt[r];
+        stt[r * 64 + lane] = st[r];
+        total += __shfl(incl, 63, 64);
+    }
+    *ok = (int64_t)total <= cap;
+    if (!*ok) return total;
+    wave_sync();
+    constexpr int U = 4;  // entry loads in flight per lane
+    for (int e0 = lane; e0 < total; e0 += U * 64) {
+        uint64_t v[U];
+        int oo[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * 64;
+            oo[u] = -1;
+            v[u] = 0;
+            if (e < total) {
+                int o = 0;  // the last k-mer whose exclusive offset is <= e
+#pragma unroll
+                for (int sft = kFlatRounds * 32; sft >= 1; sft >>= 1)
+                    if (ex[o + sft] <= e) o += sft;
+                v[u] = ix.entry[stt[o] + (e - ex[o])];
+                oo[u] = o;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (oo[u] >= 0) kbuf[e0 + u * 64] = hit_key(v[u], oo[u]);
+    }
+    wave_sync();
+    return total;
+}
+
 __device__ void load_strand(const MapArgs &a, int sidx, unsigned char *qb, int *len_out) {
     const int r = sidx >> 1, strand = sidx & 1;
     int L = a.rd.len[r];
@@ -424,14 +482,17 @@ __device__ void emit_best(const MapArgs &a, int sidx, const uint64_t *keys, int 
 __global__ void __launch_bounds__(64) map_strands(MapArgs a) {
     __shared__ uint64_t keys[kMapCap];
     __shared__ unsigned char qb[kMaxMapRead + 1];
-    __shared__ int32_t ex[64], stt[64];
+    __shared__ int32_t ex[kFlatRounds * 64], stt[kFlatRounds * 64];
     const int sidx = blockIdx.x;
     const int lane = threadIdx.x;
     int L;
     load_strand(a, sidx, qb, &L);
     __syncthreads();
     bool ok;
-    const int total = gather_hits(a.ix, qb, L - a.ix.k + 1, keys, kMapCap, ex, stt, &ok);
+    const int nq = L - a.ix.k + 1;
+    const int total = nq <= kFlatRounds * 64
+                          ? gather_hits_flat(a.ix, qb, nq, keys, kMapCap, ex, stt, &ok)
+                          : gather_hits(a.ix, qb, nq, keys, kMapCap, ex, stt, &ok);
     if (lane == 0) a.out.nhits[sidx] = total;
     if (total == 0) {
         if (lane == 0) {
@@ -562,10 +623,15 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     constexpr int PER = (kMaxMapRead + THREADS) / THREADS;
     constexpr uint32_t kEmpty = 0xFFFFFFFFu;
     constexpr uint32_t kFresh = 0x0000FFFFu;  // count 0, smallest offset 0xFFFF
+    constexpr int NS = THREADS * PER;            // k-mer slots, s = thread * PER + u
+    constexpr int HPT = kMidCap1 / THREADS;      // hits per thread (a strand here has <= kMidCap1)
+    static_assert(kMidCap1 % THREADS == 0 && kMidCap1 < 32768, "hit offsets are int16");
     __shared__ uint32_t tkey[kHashSlots], tval[kHashSlots];
     __shared__ unsigned char qb[kMaxMapRead + 1];
     __shared__ int32_t wsum[NW];
     __shared__ uint64_t red[NW];
+    __shared__ int16_t sex[NS];                  // block-wide exclusive hit offset of slot s
+    __shared__ int32_t sbase[NS];                // its first entry index minus that offset
     const int n = *list_count;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     for (int i = t; i < kHashSlots; i += THREADS) {
@@ -594,21 +660,55 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
         const int incl = wave_incl_scan(v);
         if (lane == 63) wsum[wv] = incl;
         __syncthreads();
-        int total = 0;
+        int total = 0, before = 0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) total += wsum[w];
+        for (int w = 0; w < NW; ++w) {
+            before += w < wv ? wsum[w] : 0;
+            total += wsum[w];
+        }
         if (total > kMidCap1) {
             if (t == 0) next_list[atomicAdd(next_count, 1)] = sidx;
             __syncthreads();
             continue;
         }
+        // The strand's hits, flattened over the block: hit e belongs to the
+        // last slot whose exclusive offset is <= e.  Each thread then loads
+        // its (at most HPT) entries back to back instead of walking one
+        // k-mer's bucket serially, which left the block waiting on its
+        // largest bucket, one dependent load per hit.
+        {
+            int run = before + incl - v;
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                sex[t * PER + u] = (int16_t)run;
+                sbase[t * PER + u] = st[u] - run;
+                run += c[u];
+            }
+        }
+        __syncthreads();
+        uint64_t hen[HPT];
+        int hq[HPT];
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            const int e = t + k * THREADS;
+            hen[k] = 0;
+            hq[k] = -1;
+            if (e < total) {
+                int sl = 0;
+#pragma unroll
+                for (int step = NS / 2; step >= 1; step >>= 1)
+                    if (sex[sl + step] <= e) sl += step;
+                hen[k] = a.ix.entry[sbase[sl] + e];
+                hq[k] = sl / PER + (sl % PER) * THREADS;  // the hit's query offset
+            }
+        }
         // count every hit into the table
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int o = t + u * THREADS;
-            const uint64_t *e = a.ix.entry + st[u];
-            for (int j = 0; j < c[u]; ++j) {
-                const uint64_t en = e[j];
+        for (int k = 0; k < HPT; ++k) {
+            if (hq[k] < 0) continue;
+            const int o = hq[k];
+            {
+                const uint64_t en = hen[k];
                 const uint32_t cid = a.ix.cid0[(int)(en >> 32)] +
                                      (uint32_t)((int64_t)(uint32_t)en - o + kShiftBias);
                 uint32_t h = cid_slot(cid);
