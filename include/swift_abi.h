@@ -132,7 +132,8 @@ int swift_get_coarse_rows(int *mode);
  * CU in {max, 8, 12, 16, 20, 24} x walk fused/deferred with int16 lanes, the
  * coarse row classes, and int32-only lanes) on the given pairs, reps timed
  * runs each after a warm-up, with HIP events on `stream`, then applies the
- * fastest (swift_get_coarse_rows reports its row-class mode).  Outputs are
+ * fastest (swift_get_coarse_rows reports its row-class mode: -1 when the
+ * automatic rule picks the same for a batch of this size).  Outputs are
  * written by the timed runs (identical under every setting).  table_ms
  * (optional, 5 floats per candidate: waves, int16, walk, coarse rows, ms)
  * receives up to table_cap rows; *ntable the number of candidates timed.

@@ -107,7 +107,11 @@ int swift_autotune_batch(const char *qseq_d, const int64_t *qoff_d, const int32_
     tuning().waves_per_cu = pick.waves_per_cu;
     tuning().int16 = pick.int16;
     tuning().walk_deferred = pick.walk;
-    tuning().coarse_rows = pick.coarse;
+    // the row-class choice is kept as "automatic" (-1) where the automatic
+    // rule picks the same for a batch of this size: the tuned batch gets its
+    // pick and calls of other sizes (serving slots, small batches) keep theirs
+    const int auto_coarse = npairs < kCoarsePairs ? 1 : 0;
+    tuning().coarse_rows = pick.coarse == auto_coarse ? -1 : pick.coarse;
     if (best_waves) *best_waves = pick.waves_per_cu;
     if (best_int16) *best_int16 = pick.int16;
     if (best_walk) *best_walk = pick.walk;
