@@ -176,12 +176,22 @@ def c2_leg(n_reads, steps, cpu_reads, keep=None):
         t0 = time.perf_counter()
         oix = O.Index(seqs, 12, 1000)
         t1 = time.perf_counter()
-        O.pipeline(oix, sample, mh)
+        want = O.pipeline(oix, sample, mh)
         dt = time.perf_counter() - t1
         res["cpu_baseline"] = {"value": round(len(sample) / dt, 1), "unit": "reads/s",
                                "cores": 1, "kind": "port",
                                "sample": f"first {len(sample)} reads, oracle pipeline "
                                          f"(index build {t1 - t0:.1f} s not included)"}
+        # the device pipeline on the same reads, record by record
+        got = ix.align_reads(sample, mh)
+        keys = (("read", "read"), ("strand", "strand"), ("ref_idx", "refIdx"),
+                ("ref_pos", "refPos"), ("score", "score"), ("start", "start"), ("end", "end"),
+                ("alen", "len"), ("keep", "keep"), ("qry_cons", "qry_cons"),
+                ("ref_cons", "ref_cons"))
+        same = got.n == len(want) and all(
+            got.record(i)[a] == w[b] for i, w in enumerate(want) for a, b in keys)
+        res["parity_vs_oracle"] = {"reads": len(sample), "records": len(want),
+                                   "identical": bool(same)}
     return res
 
 
@@ -391,7 +401,7 @@ def main():
         return
     if args.c2_only:
         keep = {}
-        res = c2_leg(args.c2_reads, args.steps, 0, keep)
+        res = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads, keep)
         if not args.no_c5:
             res["c5"] = c5_leg(keep["index"], keep["seqs"], args.c5_requests)
         print(json.dumps(res), flush=True)

@@ -247,3 +247,20 @@ def test_align_reads_uneven_lengths(S, orc):
         g_rec = got.record(i)
         for gk, wk in zip(REC_KEYS, ORC_KEYS):
             assert g_rec[gk] == w[wk], (i, gk)
+
+
+def test_c1_config_vs_oracle_pipeline(S, orc):
+    """The small config in full (1 Mbp uniform reference, 1,000 reads of
+    100 bp at 2 % substitutions and 0.2 % indels, k = 12, threshold 1000,
+    maxHits 3, default scoring): every record of the device pipeline equals
+    the oracle's CPU pipeline, field by field and string by string."""
+    from swift_amd import synth
+    seqs, reads, truth = synth.c1_workload()
+    ix = S.Index.from_sequences(seqs, 12, 1000)
+    got = ix.align_reads(reads, 3)
+    want = orc.pipeline(orc.Index(seqs, 12, 1000), reads, 3)
+    assert got.n == len(want) > len(reads)
+    for i, w in enumerate(want):
+        g_rec = got.record(i)
+        for gk, wk in zip(REC_KEYS, ORC_KEYS):
+            assert g_rec[gk] == w[wk], (i, gk)
