@@ -204,6 +204,16 @@ int mapHits6GetBestHits(int numBestHits, char *refIdx, int *shift, int *refPos, 
 void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName, int seedLen,
                     char *matchFile, unsigned int maxNumHits, int *numMatches, int numThreads,
                     int tupleIgnoreThres);
+/* search2Queries (src/search2.h:26-27): the reference's GPU phase-1 driver.
+ * Maps both strands of every read of qryFile and writes the same line format
+ * with the shift column 0 (src/search2.cu:341-373); maxNumHits is taken as 3
+ * whatever is passed (src/search2.cu:64).  The hits are those of
+ * searchQueries3 (lookupTable5 + mapHits6 selection): the reference's
+ * lookupTable13 kernel behind this driver draws its picks from an
+ * uninitialised generator and has no deterministic output to reproduce.
+ * Prints and exits on error, as the reference does. */
+void search2Queries(char *qryFile, char *refFile, char *matchFile, int seedLen, int maxNumHits,
+                    int *numMatches, int tupleIgnoreThres);
 
 /* ---- status-returning phase-1 API (no reference counterpart) ----------- */
 typedef struct swift_index swift_index;

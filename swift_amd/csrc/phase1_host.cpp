@@ -633,33 +633,33 @@ int mapHits6GetBestHits(int numBestHits, char *refIdx, int *shift, int *refPos, 
     return n[0];
 }
 
-void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName, int seedLen,
-                    char *matchFile, unsigned int maxNumHits, int *numMatches, int numThreads,
-                    int tupleIgnoreThres) {
-    (void)qryLen;
-    (void)numThreads;
-    // src/search.cu:279-281
-    if (qryFileName == NULL || refFileName == NULL || seedLen <= 0 || matchFile == NULL ||
-        numQrs <= 0)
-        return;
+// Phase 1 from files to a match file: index refFileName, map both strands of
+// the first numQrs reads of qryFileName, one line per hit.  zero_shift writes
+// the shift column as 0, as search2Queries' printHeuristicMatches does
+// (src/search2.cu:341-373); searchQueries3's printHeuristicMatches4 writes
+// the hit's shift (src/search.cu:2230-2240).
+static void search_to_match_file(const char *fn, const char *qryFileName, int numQrs,
+                                 const char *refFileName, int seedLen, const char *matchFile,
+                                 int maxNumHits, bool zero_shift, int *numMatches,
+                                 int tupleIgnoreThres) {
     IndexObj *ix = nullptr;
-    if (build_from_fasta(refFileName, seedLen, tupleIgnoreThres, &ix)) die(__func__);
+    if (build_from_fasta(refFileName, seedLen, tupleIgnoreThres, &ix)) die(fn);
     std::unique_ptr<IndexObj> hold(ix);
     QueryFile q;
-    if (read_query_fasta(qryFileName, &q)) die(__func__);
+    if (read_query_fasta(qryFileName, &q)) die(fn);
     const int nreads = std::min<int>(numQrs, (int)q.len.size());
     int maxlen = 0;
     for (int r = 0; r < nreads; ++r) maxlen = std::max(maxlen, q.len[r]);
     if (maxlen > kMaxMapRead) {
         set_error("reads up to %d bases (found %d)", kMaxMapRead, maxlen);
-        die(__func__);
+        die(fn);
     }
     FILE *out = fopen(matchFile, "w");
     if (!out) {
         set_error("cannot write %s", matchFile);
-        die(__func__);
+        die(fn);
     }
-    const int mh = (int)maxNumHits;
+    const int mh = maxNumHits;
     const int stride = std::max(mh, 1);
     const int batch = 1 << 18;
     int total_lines = 0;
@@ -676,16 +676,16 @@ void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName
         if (dseq.alloc(std::max<int64_t>(b1 - b0, 1)) || doff.alloc(8 * nr) ||
             dlen.alloc(4 * nr) || dref.alloc(ns * stride) || dsh.alloc(4 * ns * stride) ||
             dpos.alloc(4 * ns * stride) || dn.alloc(4 * ns))
-            die(__func__);
+            die(fn);
         if ((b1 > b0 &&
              hipMemcpy(dseq.ptr, q.bases.data() + b0, b1 - b0, hipMemcpyHostToDevice) != hipSuccess) ||
             hipMemcpy(doff.ptr, off.data(), 8 * nr, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(dlen.ptr, q.len.data() + r0, 4 * nr, hipMemcpyHostToDevice) != hipSuccess)
-            die(__func__);
+            die(fn);
         ReadsDev rd{dseq.as<char>(), doff.as<int64_t>(), dlen.as<int32_t>(), nr};
         if (map_reads(*ix, rd, maxlen, mh, dref.as<int8_t>(), dsh.as<int32_t>(), dpos.as<int32_t>(),
                       dn.as<int32_t>(), nullptr, nullptr, nullptr))
-            die(__func__);
+            die(fn);
         href.resize(ns * stride);
         hsh.resize(ns * stride);
         hpos.resize(ns * stride);
@@ -694,8 +694,7 @@ void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName
             hipMemcpy(hsh.data(), dsh.ptr, 4 * hsh.size(), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(hpos.data(), dpos.ptr, 4 * hpos.size(), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(hn.data(), dn.ptr, 4 * hn.size(), hipMemcpyDeviceToHost) != hipSuccess)
-            die(__func__);
-        // printHeuristicMatches4 (src/search.cu:2230-2240)
+            die(fn);
         for (int i = 0; i < nr; ++i) {
             const int r = r0 + i;
             for (int s = 0; s < 2; ++s) {
@@ -703,7 +702,7 @@ void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName
                 for (int h = 0; h < hn[sidx]; ++h) {
                     const size_t o = sidx * stride + h;
                     fprintf(out, "%s\t%d\t%d\t%d\t%d\t%lu\t%lu\n", q.names[r].c_str(), s,
-                            (int)href[o], hsh[o], hpos[o], q.name_off[r], q.seq_off[r]);
+                            (int)href[o], zero_shift ? 0 : hsh[o], hpos[o], q.name_off[r], q.seq_off[r]);
                     ++total_lines;
                 }
             }
@@ -711,6 +710,30 @@ void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName
     }
     fclose(out);
     if (numMatches) *numMatches = total_lines;
+}
+
+void searchQueries3(char *qryFileName, int numQrs, int qryLen, char *refFileName, int seedLen,
+                    char *matchFile, unsigned int maxNumHits, int *numMatches, int numThreads,
+                    int tupleIgnoreThres) {
+    (void)qryLen;
+    (void)numThreads;
+    // src/search.cu:279-281
+    if (qryFileName == NULL || refFileName == NULL || seedLen <= 0 || matchFile == NULL ||
+        numQrs <= 0)
+        return;
+    search_to_match_file(__func__, qryFileName, numQrs, refFileName, seedLen, matchFile,
+                         (int)maxNumHits, false, numMatches, tupleIgnoreThres);
+}
+
+void search2Queries(char *qryFile, char *refFile, char *matchFile, int seedLen, int maxNumHits,
+                    int *numMatches, int tupleIgnoreThres) {
+    (void)maxNumHits;  // src/search2.cu:64 sets it to 3 whatever is passed
+    if (qryFile == NULL || refFile == NULL || seedLen <= 0 || matchFile == NULL) {
+        set_error("search2Queries: null file name or seedLen <= 0");
+        die(__func__);
+    }
+    search_to_match_file(__func__, qryFile, INT32_MAX, refFile, seedLen, matchFile, 3, true,
+                         numMatches, tupleIgnoreThres);
 }
 
 // ---- status-returning API --------------------------------------------------

@@ -349,6 +349,15 @@ def search_queries3(qry_path, ref_path, match_path, k=12, max_hits=3, thr=1000,
     return nm.value
 
 
+def search2_queries(qry_path, ref_path, match_path, k=12, thr=1000) -> int:
+    """search2Queries (src/search2.h:26-27): every read, at most 3 hits per
+    strand, shift column written as 0."""
+    nm = C.c_int()
+    lib().search2Queries(str(qry_path).encode(), str(ref_path).encode(),
+                         str(match_path).encode(), k, 3, C.byref(nm), thr)
+    return nm.value
+
+
 def align_queries(qry_path, ref_path, match_path, out_path, qry_len, out_format=0,
                   scoring=Scoring(), num_matches=None, gpu_driver=True) -> None:
     """Phase 2 from the match file, as the reference's drivers run it:

@@ -199,6 +199,47 @@ def test_search_queries3_known(S, case, tmp_path):
         assert nm == case["num_matches"]
 
 
+def test_search2_queries_vs_oracle(S, orc, tmp_path):
+    """search2Queries: every read, both strands, at most 3 hits each, the
+    shift column 0 (src/search2.cu:64,341-373).  Expected lines come from the
+    oracle's per-strand mapping, with the name and offsets of each read's
+    header line (qryGetNext2, src/query.cu:174-215: nameOffset at the '>',
+    seqOffset just past the header line)."""
+    from swift_amd import synth
+    rnd = random.Random(41)
+    seqs = rand_genome(rnd, 3, 3000, 20000, repeats=6)
+    ref = tmp_path / "ref.fa"
+    with open(ref, "wb") as f:
+        for i, s in enumerate(seqs):
+            f.write(b">chr%d\n" % i + b"".join(s[p:p + 50] + b"\n" for p in range(0, len(s), 50)))
+    g = np.random.default_rng(41)
+    reads, _ = synth.sample_reads(g, [np.frombuffer(s, np.uint8) for s in seqs], 80, 100,
+                                  0.02, 0.002)
+    reads += [bytes(rnd.choice(b"ACGT") for _ in range(100)) for _ in range(5)]
+    reads += [seqs[2][:70] + b"NNNN" + seqs[2][74:100], b"ACGT" * 3]
+    qry = tmp_path / "reads.fa"
+    blob = bytearray()
+    want = []
+    ix = orc.Index(seqs, 12, 1000)
+    for i, r in enumerate(reads):
+        name = b"read %d x" % i
+        name_off = len(blob)
+        blob += b">" + name + b"\n"
+        seq_off = len(blob)
+        blob += r + b"\n"
+        for strand, s in enumerate((r, orc.revcomp(r))):
+            hits, _, _ = ix.map_query(s, 3)
+            want += ["%s\t%d\t%d\t0\t%d\t%d\t%d" % (name.decode(), strand, h[0], h[2],
+                                                     name_off, seq_off) for h in hits]
+    qry.write_bytes(bytes(blob))
+    out = tmp_path / "m.txt"
+    nm = S.search2_queries(qry, ref, out, 12, 1000)
+    lines = out.read_text().splitlines()
+    assert len(want) > 80
+    assert nm == len(lines)
+    assert lines == want
+
+
 # ------------------------------------------------------------------ pipeline
 REC_KEYS = ("read", "strand", "ref_idx", "shift", "ref_pos", "nbiggest", "ref_distance", "score",
             "start", "end", "alen", "keep", "ntied", "qry_cons", "ref_cons")
