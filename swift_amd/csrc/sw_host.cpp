@@ -400,7 +400,12 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     };
     KeyConsts kc16 = make_consts(sc, kOneHot), kc32 = make_consts(sc, kBytes);
     int pos = std::max(std::max(std::max(sc.match, sc.mismatch), std::max(sc.gap_open, sc.gap_ext)), 0);
-    uint32_t h_limit = (uint32_t)(32767 - (4 * pos + 3) - 4);
+    // int16 lanes: a best score at this limit flags the pair for the int32
+    // pass.  The keys 4H + direction stay below 32768 and the best-cell keys
+    // 32H + 31 below 65536 for every cell of an unflagged pair: H rises by
+    // at most pos per cell, so a pair with any larger cell has one in
+    // [2048 - pos, 2047] whose key is exact.
+    uint32_t h_limit = (uint32_t)std::min(32767 - (4 * pos + 3) - 4, 4 * (2048 - pos));
 
     // results start at "score 0": a pair no kernel reaches never walks
     SWIFT_HIP_TRY(launch_fill(w.res.as<int32_t>(), n * (int)(sizeof(PairResult) / 4), 0, st));

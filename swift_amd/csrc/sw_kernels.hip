@@ -154,11 +154,15 @@ struct Sweep {
     uint32_t q2[PK ? R : 1];   // PK: match tables of half 1
     uint32_t dr[R];       // DR of (row, j-1)
     uint32_t el[R];       // EL of (row, j-1)
-    uint32_t best[R];     // int32: H<<12 | (4095-j) ; PK: row max of 4H
-    uint32_t bcol[R];     // PK only: first column of the row max (both halves)
+    // int32: best[r] = max over the row of H<<12 | (4095-j).
+    // PK: best[r] = running max of the key 32H + (15 - t%16) (unsigned),
+    // sealed to 32H + 31 at each block end; bcol[r] = block << 5 | key code
+    // of the row's first strict maximum (see block_end).
+    uint32_t best[R];
+    uint32_t bcol[R];
     uint32_t eu_last, dr_last, dr_top;
 
-    uint32_t c_dr, c_dsub, c_eo2, c_eo1, c_de, c_sign;
+    uint32_t c_dr, c_dsub, c_eo2, c_eo1, c_de, c_sign, c_eight;
     // PK: 16-bit tables indexed by direction code, laid out for v_perm_b32
     // (entry of code c in bytes 2c, 2c+1): the up exit, and the left exit's
     // penalty (subtracted with saturation at 0)
@@ -177,6 +181,7 @@ struct Sweep {
         c_eo1 = splat<PK>(k.eo1);
         c_de = splat<PK>(k.de);
         c_sign = splat<PK>(k.sign);
+        c_eight = opaque(splat<PK>(8));  // kept a multiplier: one v_pk_mad per key
         // up exit: open unless the cell came from above (code 2); left exit:
         // open unless it came from the left (code 1)
         lut(k.eo2, k.eo2, k.eo2 + k.de, k.eo2, lut_u_lo, lut_u_hi);
@@ -186,7 +191,7 @@ struct Sweep {
             dr[r] = c_dr;   // column 0: H = 0, start
             // column 0 is not a left move -> open; PK: clamped at 0
             el[r] = PK ? 0u : c_eo1;
-            best[r] = 0;
+            best[r] = PK ? 0x001F001Fu : 0u;  // PK: H = 0, sealed
             bcol[r] = 0;
         }
         eu_last = c_eo2;
@@ -195,9 +200,11 @@ struct Sweep {
     }
 
     // One step.  rr: reference code(s) of column j; j = t - g; vmask: per
-    // element all-ones where column j is inside the pair (MASKED only).
+    // element all-ones where column j is inside the pair (MASKED only);
+    // code (PK): 15 - t%16 in both halves.
     template <bool MASKED>
-    __device__ __forceinline__ void step(uint32_t rr, int j, uint32_t vmask, uint32_t *acc) {
+    __device__ __forceinline__ void step(uint32_t rr, int j, uint32_t vmask, uint32_t code,
+                                         uint32_t *acc) {
         // row 0 of this lane takes the lane above's last row: its up exit at
         // column j (computed last step) and its diagonal value at column j-1
         // (received last step).  Lane 0 of the group sees matrix row 0.
@@ -207,22 +214,29 @@ struct Sweep {
         dr_top = dr_next;
         if (MASKED && j < 0) return;  // left of column 1: keep column-0 state
 
-        uint32_t jj = 0, colterm = 0;
-        if constexpr (PK) jj = splat<true>(j);
-        else colterm = (uint32_t)(kMaxCols - j);
+        uint32_t colterm = 0;
+        if constexpr (!PK) colterm = (uint32_t)(kMaxCols - j);
         // The diagonal, left and zero candidates depend only on column j-1:
         // fold them for every row first, so the row-to-row chain through the
         // up exit eu is just max -> code -> exit per row, with the rows'
         // independent work free to fill its gaps (max is order-free: the
         // direction code in the key's low bits breaks ties).
         uint32_t pre[R];
+        if constexpr (PK) {
+            // all diagonal candidates first, then the maxes: a packed result
+            // read by the very next instruction costs a wait state
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t diag = r == 0 ? diag0 : dr[r - 1];
-            if constexpr (PK) {
+            for (int r = 0; r < R; ++r) {
+                const uint32_t diag = r == 0 ? diag0 : dr[r - 1];
                 const uint32_t f = __builtin_amdgcn_perm(q2[r], q[r], rr);  // 0xFFFF: match
-                pre[r] = vmax<true>(vmad<true>(f, c_dsub, diag), el[r]);    // el >= 0
-            } else {
+                pre[r] = vmad<true>(f, c_dsub, diag);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) pre[r] = vmax<true>(pre[r], el[r]);  // el >= 0
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t diag = r == 0 ? diag0 : dr[r - 1];
                 const uint32_t dk = vmad<false>(byte_mismatch(q[r], rr), c_dsub, diag);
                 pre[r] = vmax<false>(vmax<false>(dk, el[r]), 0u);
             }
@@ -247,10 +261,10 @@ struct Sweep {
             }
             if constexpr (PK) {
                 static_assert(!(PK && MASKED), "packed steps are never masked");
-                uint32_t d = vsub<true>(best[r], h4);              // < 0 <=> new strict max
-                uint32_t m = as_u(as_s(d) >> as_s(c_sign));        // 0xFFFF per new max
-                bcol[r] = (m & jj) | (~m & bcol[r]);
-                best[r] = vmax<true>(best[r], h4);
+                // key = 32H + code: within a block of 16 steps the largest
+                // key is the row's largest H at its earliest step
+                best[r] = as_u(__builtin_elementwise_max(
+                    as_us(best[r]), as_us(vmad<true>(h4, c_eight, code))));
             } else {
                 uint32_t key = (h4 << (kColBits - 2)) | colterm;
                 if (MASKED) key &= vmask;
@@ -259,6 +273,23 @@ struct Sweep {
         }
         eu_last = eu;
         dr_last = dr[R - 1];
+    }
+
+    // PK, after the last step of a block of 16 (and after the sweep's last
+    // step).  The previous block end sealed the running maximum to 32H + 31,
+    // which every later key of the same H (code <= 15) leaves in place; a key
+    // of larger H replaces it.  So a row's maximum rose in this block exactly
+    // where the low 5 bits are no longer 31, and the row's first strict
+    // maximum is then in this block, at the key's code.  blk32 = block index
+    // << 5 in both halves.
+    __device__ __forceinline__ void block_end(uint32_t blk32) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t lo = best[r] & 0x001F001Fu;
+            const uint32_t m = as_u(as_s(vsub<true>(lo, 0x001F001Fu)) >> as_s(c_sign));
+            bcol[r] = (m & (lo | blk32)) | (~m & bcol[r]);
+            best[r] |= 0x001F001Fu;
+        }
     }
 };
 
@@ -474,7 +505,7 @@ sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, in
         uint32_t a1[S::W];
 #pragma unroll
         for (int w = 0; w < S::W; ++w) a1[w] = 0;
-        sw.template step<decltype(masked)::value>(rr, j, vmask, a1);
+        sw.template step<decltype(masked)::value>(rr, j, vmask, 0u, a1);
 #pragma unroll
         for (int w = 0; w < S::W; ++w) tr[trace_word<S::W>(t, w, g)] = a1[w];
     };
@@ -600,7 +631,7 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
         uint32_t a1[S::W];
 #pragma unroll
         for (int w = 0; w < S::W; ++w) a1[w] = 0;
-        sw.template step<decltype(masked)::value>(rr, j, vmask, a1);
+        sw.template step<decltype(masked)::value>(rr, j, vmask, splat<PK>(15 - (t & 15)), a1);
 #pragma unroll
         for (int w = 0; w < S::W; ++w) tr[trace_word<S::W>(t, w, lane)] = a1[w];
     };
@@ -608,7 +639,10 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     // row maxima.  Packed: no step is masked (see Sweep).
     const int T = tk.steps;
     if constexpr (PK) {
-        for (int t = 0; t < T; ++t) run(std::false_type{}, t);
+        for (int t = 0; t < T; ++t) {
+            run(std::false_type{}, t);
+            if ((t & 15) == 15 || t == T - 1) sw.block_end(splat<true>((t >> 4) << 5));
+        }
     } else {
         const int lo = min(kBatchG - 1, T), hi = max(lo, tk.min_rlen);
         sweep_steps(T, lo, hi, run);
@@ -624,8 +658,10 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
             int row = g * R + r;
             uint32_t hv, cv;
             if constexpr (PK) {
-                hv = ((sw.best[r] >> (16 * h)) & 0xFFFFu) >> 2;
-                cv = ((sw.bcol[r] >> (16 * h)) & 0xFFFFu) + 1;
+                // block << 5 | (15 - t%16) -> step t -> column t - g (+1)
+                const uint32_t bc = (sw.bcol[r] >> (16 * h)) & 0xFFFFu;
+                hv = ((sw.best[r] >> (16 * h)) & 0xFFFFu) >> 5;
+                cv = ((bc >> 5) << 4) + 15u - (bc & 15u) - (uint32_t)g + 1;
             } else {
                 hv = sw.best[r] >> kColBits;
                 cv = kMaxCols - (sw.best[r] & kMaxCols) + 1;

@@ -145,6 +145,36 @@ def test_batch_int16_overflow_rerun(A, orc):
         compare(res.record(i), orc.sw(rs[i], qs[i], 200, -1, -10, -1), i)
 
 
+@pytest.mark.parametrize("match", [7, 8, 9])
+def test_batch_int16_key_range(A, orc, match):
+    """Best scores up to about 2,900: the int16 lanes keep best cells as keys
+    32H + code (unsigned, exact while H <= 2047, past 32767 from H = 1024
+    on) and flag a pair whose best reaches 2048 - match for the int32 pass.
+    Exact matches of 120-320 bases put the bests on both sides of that
+    limit, mixed with ordinary pairs in the same tasks."""
+    rnd = random.Random(match)
+    qs, rs = [], []
+    for i in range(240):
+        L = rnd.randint(120, 320) if i % 2 == 0 else rnd.randint(30, 200)
+        reg = bytes(rnd.choice(b"ACGT") for _ in range(L + rnd.randint(0, 150)))
+        o = rnd.randint(0, len(reg) - L)
+        q = bytearray(reg[o:o + L])
+        if i % 4 == 1:
+            for _ in range(3):
+                q[rnd.randrange(L)] = rnd.choice(b"ACGT")
+        qs.append(bytes(q))
+        rs.append(reg)
+    sc = (match, -1, -10, -1)
+    res = A.sw_batch(qs, rs, A.Scoring(*sc))
+    want = [orc.sw(rs[i], qs[i], *sc) for i in range(len(qs))]
+    lim = 2048 - match
+    assert any(1024 < w["score"] < lim for w in want)   # keys past 32767, kept in int16
+    assert any(w["score"] >= lim for w in want)         # flagged for the int32 pass
+    assert A.batch_stats()["nrerun"] > 0
+    for i in range(len(qs)):
+        compare(res.record(i), want[i], (i, sc))
+
+
 @pytest.mark.parametrize("sc", SCORINGS)
 def test_batch_scorings_vs_oracle(A, orc, sc):
     rnd = random.Random(sum(sc) + 100)
