@@ -182,16 +182,61 @@ def c2_leg(n_reads, steps, cpu_reads, keep=None):
                                "cores": 1, "kind": "port",
                                "sample": f"first {len(sample)} reads, oracle pipeline "
                                          f"(index build {t1 - t0:.1f} s not included)"}
-        # the device pipeline on the same reads, record by record
-        got = ix.align_reads(sample, mh)
-        keys = (("read", "read"), ("strand", "strand"), ("ref_idx", "refIdx"),
-                ("ref_pos", "refPos"), ("score", "score"), ("start", "start"), ("end", "end"),
-                ("alen", "len"), ("keep", "keep"), ("qry_cons", "qry_cons"),
-                ("ref_cons", "ref_cons"))
-        same = got.n == len(want) and all(
-            got.record(i)[a] == w[b] for i, w in enumerate(want) for a, b in keys)
-        res["parity_vs_oracle"] = {"reads": len(sample), "records": len(want),
-                                   "identical": bool(same)}
+        res["parity_vs_oracle"] = pipeline_parity(ix, sample, want, mh)
+    return res
+
+
+def pipeline_parity(ix, reads, want, mh):
+    """The device pipeline on `reads` against the oracle pipeline's records
+    `want`, record by record, strings included."""
+    got = ix.align_reads(reads, mh)
+    keys = (("read", "read"), ("strand", "strand"), ("ref_idx", "refIdx"),
+            ("ref_pos", "refPos"), ("score", "score"), ("start", "start"), ("end", "end"),
+            ("alen", "len"), ("keep", "keep"), ("qry_cons", "qry_cons"),
+            ("ref_cons", "ref_cons"))
+    same = got.n == len(want) and all(
+        got.record(i)[a] == w[b] for i, w in enumerate(want) for a, b in keys)
+    return {"reads": len(reads), "records": len(want), "identical": bool(same)}
+
+
+def c1_leg(steps, cpu):
+    """The small config (1 Mbp uniform reference, 1,000 reads of 100 bp at 2 %
+    substitutions and 0.2 % indels, k=12, thr 1000, maxHits 3): reads/s of the
+    whole device pipeline, and the oracle's CPU pipeline over the same reads
+    timed beside it and compared record by record."""
+    from swift_amd import search, synth
+    from swift_amd.device import synchronize
+    seqs, reads, _ = synth.c1_workload()
+    t0 = time.perf_counter()
+    ix = search.Index.from_sequences(seqs, 12, 1000)
+    build_s = time.perf_counter() - t0
+    rb = search.ReadBatch(reads)
+    mh = 3
+    out = search.RecordBuffers(2 * rb.n * mh, rb.max_len)
+    ix.align_batch(rb, out, mh)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(steps, 1) * 4):
+        ix.align_batch(rb, out, mh)
+    synchronize()
+    pipe_s = (time.perf_counter() - t0) / (max(steps, 1) * 4)
+    res = {"workload": "C1: 1 Mbp uniform reference, 1000 reads of 100 bp (2 % subs, 0.2 % "
+                       "indels), k=12, thr 1000, maxHits 3",
+           "index_build_s": round(build_s, 3),
+           "pipeline_reads_per_s": round(len(reads) / pipe_s, 1),
+           "pipeline_ms": round(pipe_s * 1e3, 3), "records": out.results(strings=False).n}
+    if cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        oix = O.Index(seqs, 12, 1000)
+        t1 = time.perf_counter()
+        want = O.pipeline(oix, reads, mh)
+        dt = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": round(len(reads) / dt, 1), "unit": "reads/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"all {len(reads)} reads, oracle pipeline "
+                                         f"(index build {t1 - t0:.2f} s not included)"}
+        res["parity_vs_oracle"] = pipeline_parity(ix, reads, want, mh)
     return res
 
 
@@ -350,6 +395,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=800000)
     ap.add_argument("--c2-reads", type=int, default=100000)
     ap.add_argument("--c2-cpu-reads", type=int, default=3000)
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-requests", type=int, default=400)
@@ -503,6 +549,9 @@ def main():
                                  "calls": len(groups)}
         out["speedup_vs_plain"] = round(gcups / (cells / pdt / 1e9), 3)
         del groups
+    if not args.no_c1:
+        out["c1"] = c1_leg(args.steps, not args.no_cpu)
+        log(f"[bench] C1: {out['c1']}")
     if not args.no_c2:
         keep = {}
         out["c2"] = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads,
