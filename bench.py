@@ -317,7 +317,44 @@ def c5_leg(index, seqs, n_requests, seed=20261205):
     return res
 
 
-def c4_leg(n_reads, batch, parity_reads, c5_requests=0):
+def c4_graph(ix, g, off, lens, rng, n_reads, batch):
+    """The C4 pipeline as one hipGraph replay per batch: `batch` fixed read
+    slots of 150 bases, each batch uploaded into them (upload timed, as for
+    the eager batches), and a 20,000-read batch compared record by record,
+    strings included, with the eager pipeline."""
+    from swift_amd import search, synth
+    from swift_amd._lib import check, lib
+    from swift_amd.device import synchronize
+    pg = search.PipelineGraph(ix, batch, 150, 3)
+    dt, nrec, done = 0.0, 0, 0
+    while done < n_reads:
+        nb = min(batch, n_reads - done)
+        buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, nb, 150, 0.01)
+        assert np.array_equal(ro, np.arange(nb, dtype=np.int64) * 150)
+        slot_len = np.zeros(batch, np.int32)
+        slot_len[:nb] = rl
+        synchronize()
+        t0 = time.perf_counter()
+        pg.seq.upload(buf)
+        pg.len.upload(slot_len)
+        pg.launch()
+        synchronize()
+        dt += time.perf_counter() - t0
+        check(lib().swift_pipeline_graph_check(pg._h))
+        nrec += int(pg.out.bufs["nrecords"].download(np.int32, 1)[0])
+        done += nb
+        log(f"[bench] C4 graph {done}/{n_reads} reads, {dt:.2f} s on the device path")
+    buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, 20000, 150, 0.01)
+    reads = [buf[o:o + 150].tobytes() for o in ro]
+    a = pg.run(reads)
+    b = ix.align_reads(reads, 3)
+    same = a.n == b.n and all(a.record(i) == b.record(i) for i in range(a.n))
+    pg.close()
+    return {"reads_per_s": round(n_reads / dt, 1), "records": nrec,
+            "vs_eager": {"reads": len(reads), "records": a.n, "identical": bool(same)}}
+
+
+def c4_leg(n_reads, batch, parity_reads, c5_requests=0, graph=True):
     """Genome scale: 3.1 Gbp / 25 sequences (GC 41 %, 45 % interspersed
     repeats), k=12 index resident in HBM, 10M reads of 150 bp at 1 %
     substitutions streamed through the device pipeline in batches.  Reports
@@ -360,6 +397,8 @@ def c4_leg(n_reads, batch, parity_reads, c5_requests=0):
            "index_build_s": round(build_s, 2), "kept_tuples": info["total"],
            "largest_bucket": info["max_bucket"], "reads_per_s": round(n_reads / dt, 1),
            "records": nrec, "batch_reads": batch, "hits_per_strand": hit_stats}
+    if graph:
+        res["graph"] = c4_graph(ix, g, off, lens, rng, n_reads, batch)
     if parity_reads > 0:
         from oracle import oracle as O
         buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, parity_reads, 150, 0.01)
