@@ -591,10 +591,13 @@ map_strands_wg(MapArgs a, const int32_t *list, const int32_t *list_count, int32_
 // Hash tier (strands of kMapCap < hits <= kMidCap1 when 32-bit cluster ids
 // fit): no sort.  Each hit is counted into an LDS open-addressing table keyed
 // by its cluster id (count and smallest query offset packed in one word,
-// updated by compare-and-swap); workgroup reductions then give the biggest
-// count, the number of clusters of that count, and the first max_hits such
-// clusters in cluster-id order, i.e. the (refIdx, shift) order of
-// findBiggestClusters with the first element of each cluster.
+// updated by compare-and-swap); the inserting threads keep each hit's slot
+// and the largest count they produced.  After the biggest count is known,
+// every cluster of that count is listed once, through its hit at the
+// cluster's smallest query offset, and one wave picks the first max_hits of
+// them in cluster-id order, i.e. the (refIdx, shift) order of
+// findBiggestClusters with the first element of each cluster.  The threads
+// then clear only the slots their hits used.
 __device__ __forceinline__ uint32_t cid_slot(uint32_t cid) {
     return (cid * 2654435761u) >> (32 - 13);  // log2(kHashSlots) = 13
 }
@@ -615,6 +618,12 @@ __device__ __forceinline__ uint64_t block_min_u64(uint64_t v, uint64_t *red) {
     return m;
 }
 
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, s, 64));
+    return v;
+}
+
 template <int THREADS>
 __global__ void __launch_bounds__(THREADS)
 map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int32_t *next_list,
@@ -625,6 +634,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     constexpr uint32_t kFresh = 0x0000FFFFu;  // count 0, smallest offset 0xFFFF
     constexpr int NS = THREADS * PER;            // k-mer slots, s = thread * PER + u
     constexpr int HPT = kMidCap1 / THREADS;      // hits per thread (a strand here has <= kMidCap1)
+    constexpr int kCand = 64;                    // biggest clusters picked by one wave
     static_assert(kMidCap1 % THREADS == 0 && kMidCap1 < 32768, "hit offsets are int16");
     __shared__ uint32_t tkey[kHashSlots], tval[kHashSlots];
     __shared__ unsigned char qb[kMaxMapRead + 1];
@@ -632,6 +642,9 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     __shared__ uint64_t red[NW];
     __shared__ int16_t sex[NS];                  // block-wide exclusive hit offset of slot s
     __shared__ int32_t sbase[NS];                // its first entry index minus that offset
+    __shared__ uint64_t cand[kCand];             // keys of the biggest clusters
+    __shared__ uint32_t sh_best;
+    __shared__ int32_t sh_ncand;
     const int n = *list_count;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     for (int i = t; i < kHashSlots; i += THREADS) {
@@ -641,6 +654,10 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const int sidx = list[it];
         int L;
+        if (t == 0) {
+            sh_best = 0;
+            sh_ncand = 0;
+        }
         load_strand(a, sidx, qb, &L);
         __syncthreads();
         const int nq = L - a.ix.k + 1;
@@ -687,12 +704,13 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
         }
         __syncthreads();
         uint64_t hen[HPT];
-        int hq[HPT];
+        int hq[HPT], hh[HPT];
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             const int e = t + k * THREADS;
             hen[k] = 0;
             hq[k] = -1;
+            hh[k] = 0;
             if (e < total) {
                 int sl = 0;
 #pragma unroll
@@ -702,82 +720,110 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
                 hq[k] = sl / PER + (sl % PER) * THREADS;  // the hit's query offset
             }
         }
-        // count every hit into the table
+        // count every hit into the table; remember its slot and the largest
+        // count this thread produced
+        uint32_t mybest = 0;
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hq[k] < 0) continue;
             const int o = hq[k];
-            {
-                const uint64_t en = hen[k];
-                const uint32_t cid = a.ix.cid0[(int)(en >> 32)] +
-                                     (uint32_t)((int64_t)(uint32_t)en - o + kShiftBias);
-                uint32_t h = cid_slot(cid);
-                for (;;) {
-                    const uint32_t old = atomicCAS(&tkey[h], kEmpty, cid);
-                    if (old == kEmpty || old == cid) break;
-                    h = (h + 1) & (kHashSlots - 1);
-                }
-                uint32_t cur = tval[h], seen;
-                do {
-                    seen = cur;
-                    const uint32_t cnt = (seen >> 16) + 1;
-                    const uint32_t mo = min(seen & 0xFFFFu, (uint32_t)o);
-                    cur = atomicCAS(&tval[h], seen, (cnt << 16) | mo);
-                } while (cur != seen);
+            const uint64_t en = hen[k];
+            const uint32_t cid = a.ix.cid0[(int)(en >> 32)] +
+                                 (uint32_t)((int64_t)(uint32_t)en - o + kShiftBias);
+            uint32_t h = cid_slot(cid);
+            for (;;) {
+                const uint32_t old = atomicCAS(&tkey[h], kEmpty, cid);
+                if (old == kEmpty || old == cid) break;
+                h = (h + 1) & (kHashSlots - 1);
+            }
+            uint32_t cur = tval[h], seen, cnt;
+            do {
+                seen = cur;
+                cnt = (seen >> 16) + 1;
+                const uint32_t mo = min(seen & 0xFFFFu, (uint32_t)o);
+                cur = atomicCAS(&tval[h], seen, (cnt << 16) | mo);
+            } while (cur != seen);
+            mybest = max(mybest, cnt);
+            hh[k] = (int)h;
+        }
+        mybest = wave_max_u32(mybest);
+        if (lane == 0) atomicMax(&sh_best, mybest);
+        __syncthreads();
+        const uint32_t best = sh_best;
+        // The clusters of the biggest count, each through its hit at the
+        // cluster's smallest query offset (one hit per cluster: two hits of
+        // one cluster at the same offset would be the same reference position).
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            if (hq[k] < 0) continue;
+            const uint32_t tv = tval[hh[k]];
+            if ((tv >> 16) == best && (int)(tv & 0xFFFFu) == hq[k]) {
+                const int at = atomicAdd(&sh_ncand, 1);
+                if (at < kCand) cand[at] = ((uint64_t)tkey[hh[k]] << 16) | (uint32_t)hq[k];
             }
         }
         __syncthreads();
-        // biggest count and the number of clusters that reach it
-        uint32_t best = 0;
-        for (int i = t; i < kHashSlots; i += THREADS) best = max(best, tval[i] >> 16);
-        best = (uint32_t)(~block_min_u64<THREADS>(~(uint64_t)best, red));
-        int nbig = 0;
-        for (int i = t; i < kHashSlots; i += THREADS) nbig += (tval[i] >> 16) == best;
-        const int nbig_w = wave_incl_scan(nbig);
-        if (lane == 63) wsum[wv] = nbig_w;
-        __syncthreads();
-        int nbig_all = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) nbig_all += wsum[w];
+        const int nbig_all = sh_ncand;
         // the first max_hits of them by cluster id
         const int take = min(nbig_all, a.max_hits);
         const int64_t ob = (int64_t)sidx * a.out.stride;
-        uint64_t last = 0;
-        for (int r = 0; r < take; ++r) {
-            uint64_t mine = ~0ull;
-            for (int i = t; i < kHashSlots; i += THREADS) {
-                if ((tval[i] >> 16) != best) continue;
-                const uint64_t k = ((uint64_t)tkey[i] << 16) | (tval[i] & 0xFFFFu);
-                if ((r == 0 || k > last) && k < mine) mine = k;
-            }
-            const uint64_t k = block_min_u64<THREADS>(mine, red);
-            last = k;
-            if (t == 0) {
-                const uint32_t cid = (uint32_t)(k >> 16);
-                int ref = 0;  // last sequence whose cid0 <= cid
-                for (int lo = 0, hi = a.ix.nseq - 1; lo <= hi;) {
-                    const int mid = (lo + hi) >> 1;
-                    if (a.ix.cid0[mid] <= cid) {
-                        ref = mid;
-                        lo = mid + 1;
-                    } else {
-                        hi = mid - 1;
-                    }
+        auto emit = [&](int r, uint64_t k) {
+            const uint32_t cid = (uint32_t)(k >> 16);
+            int ref = 0;  // last sequence whose cid0 <= cid
+            for (int lo = 0, hi = a.ix.nseq - 1; lo <= hi;) {
+                const int mid = (lo + hi) >> 1;
+                if (a.ix.cid0[mid] <= cid) {
+                    ref = mid;
+                    lo = mid + 1;
+                } else {
+                    hi = mid - 1;
                 }
-                const int sh = (int)((int64_t)(cid - a.ix.cid0[ref]) - kShiftBias);
-                a.out.ref[ob + r] = (int8_t)ref;
-                a.out.shift[ob + r] = sh;
-                a.out.pos[ob + r] = sh + (int)(k & 0xFFFFu);
+            }
+            const int sh = (int)((int64_t)(cid - a.ix.cid0[ref]) - kShiftBias);
+            a.out.ref[ob + r] = (int8_t)ref;
+            a.out.shift[ob + r] = sh;
+            a.out.pos[ob + r] = sh + (int)(k & 0xFFFFu);
+        };
+        if (nbig_all <= kCand) {
+            if (wv == 0) {  // one wave: repeated wave minimum over the list
+                uint64_t kk = lane < nbig_all ? cand[lane] : ~0ull;
+                for (int r = 0; r < take; ++r) {
+                    uint64_t m = kk;
+#pragma unroll
+                    for (int s2 = 32; s2 >= 1; s2 >>= 1) {
+                        const uint64_t o2 = __shfl_xor(m, s2, 64);
+                        m = o2 < m ? o2 : m;
+                    }
+                    if (lane == 0) emit(r, m);
+                    if (kk == m) kk = ~0ull;  // cluster ids are distinct
+                }
+            }
+        } else {
+            // more tied clusters than the list holds: scan the table
+            uint64_t last = 0;
+            for (int r = 0; r < take; ++r) {
+                uint64_t mine = ~0ull;
+                for (int i = t; i < kHashSlots; i += THREADS) {
+                    if ((tval[i] >> 16) != best) continue;
+                    const uint64_t k = ((uint64_t)tkey[i] << 16) | (tval[i] & 0xFFFFu);
+                    if ((r == 0 || k > last) && k < mine) mine = k;
+                }
+                const uint64_t k = block_min_u64<THREADS>(mine, red);
+                last = k;
+                if (t == 0) emit(r, k);
             }
         }
         if (t == 0) {
             a.out.nbest[sidx] = take;
             a.out.nbig[sidx] = nbig_all;
         }
-        __syncthreads();
-        for (int i = t; i < kHashSlots; i += THREADS) {
-            tkey[i] = kEmpty;
-            tval[i] = kFresh;
+        // every table read of this strand is behind a barrier: clear the
+        // slots this thread's hits used
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            if (hq[k] < 0) continue;
+            tkey[hh[k]] = kEmpty;
+            tval[hh[k]] = kFresh;
         }
         __syncthreads();
     }
