@@ -588,7 +588,7 @@ map_strands_wg(MapArgs a, const int32_t *list, const int32_t *list_count, int32_
     }
 }
 
-// Hash tier (strands of kMapCap < hits <= kMidCap1 when 32-bit cluster ids
+// Hash tier (strands of kMapCap < hits <= kHashCap when 32-bit cluster ids
 // fit): no sort.  Each hit is counted into an LDS open-addressing table keyed
 // by its cluster id (count and smallest query offset packed in one word,
 // updated by compare-and-swap); the inserting threads keep each hit's slot
@@ -633,9 +633,10 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     constexpr uint32_t kEmpty = 0xFFFFFFFFu;
     constexpr uint32_t kFresh = 0x0000FFFFu;  // count 0, smallest offset 0xFFFF
     constexpr int NS = THREADS * PER;            // k-mer slots, s = thread * PER + u
-    constexpr int HPT = kMidCap1 / THREADS;      // hits per thread (a strand here has <= kMidCap1)
+    constexpr int HPT = kHashCap / THREADS;      // hits per thread (a strand here has <= kHashCap)
     constexpr int kCand = 64;                    // biggest clusters picked by one wave
-    static_assert(kMidCap1 % THREADS == 0 && kMidCap1 < 32768, "hit offsets are int16");
+    static_assert(kHashCap % THREADS == 0 && kHashCap < 32768, "hit offsets are int16");
+    static_assert(4 * kHashCap <= 3 * kHashSlots, "distinct clusters fill <= 3/4 of the table");
     __shared__ uint32_t tkey[kHashSlots], tval[kHashSlots];
     __shared__ unsigned char qb[kMaxMapRead + 1];
     __shared__ int32_t wsum[NW];
@@ -683,7 +684,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             before += w < wv ? wsum[w] : 0;
             total += wsum[w];
         }
-        if (total > kMidCap1) {
+        if (total > kHashCap) {
             if (t == 0) next_list[atomicAdd(next_count, 1)] = sidx;
             __syncthreads();
             continue;

@@ -16,8 +16,9 @@ constexpr int kMaxMapRead = 2047;      // query offsets fit 11 bits
 constexpr int64_t kShiftBias = 2048;   // shift >= -(kMaxMapRead - 1)
 constexpr int kMapCap = 1024;          // hits per strand sorted in LDS by one wave
 constexpr int kMidCap1 = 4096;         // ... by one 256-thread workgroup (32 KB LDS)
-constexpr int kHashSlots = 8192;       // cluster table of the hash tier (64 KB LDS):
-                                       // strands of <= kMidCap1 hits
+constexpr int kHashSlots = 8192;       // cluster table of the hash tier (64 KB LDS)
+constexpr int kHashCap = 6144;         // hits per strand the hash tier counts: table load
+                                       // <= 0.75, so every insertion finds its slot
 constexpr int kMidCap = 16384;         // ... by one 1024-thread workgroup (128 KB LDS)
 constexpr int kSmallBucket = 32;       // buckets sorted by one thread
 constexpr int kWaveBucket = 1024;      // buckets sorted by one wave in LDS
@@ -85,7 +86,8 @@ struct MapArgs {
     int64_t scratch_cap;
     int scratch_slots;
     int32_t *ovl_list, *ovl_count;   // strands past kMapCap hits
-    int32_t *mid_list, *mid_count;   // strands past kMidCap1 hits
+    int32_t *mid_list, *mid_count;   // strands past the hash tier (kHashCap hits, or
+                                     // kMidCap1 without 32-bit cluster ids)
     int32_t *big_list, *big_count;   // strands past kMidCap hits
     int mid_blocks;                  // workgroups of map_strands_mid (0: none needed)
     int32_t *error;           // bit 0: a strand exceeded scratch_cap, bit 1: a read
