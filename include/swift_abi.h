@@ -216,6 +216,16 @@ void search2Queries(char *qryFile, char *refFile, char *matchFile, int seedLen, 
                     int *numMatches, int tupleIgnoreThres);
 
 /* ---- status-returning phase-1 API (no reference counterpart) ----------- */
+
+/* Which mapping tier a strand starts in.  -1 (default): the hash-cluster
+ * tier when the index's mean bucket size times the k-mers of a
+ * max_read_len read reaches 1024 hits (the wave tier would only count such
+ * strands and pass them on), else the one-wave tier; 0: always the one-wave
+ * tier first; 1: always the hash-cluster tier first (when the index has
+ * 32-bit cluster ids).  Results are identical in every mode. */
+int swift_set_map_mode(int mode);
+int swift_get_map_mode(int *mode);
+
 typedef struct swift_index swift_index;
 
 /* Index over nseq sequences in host memory (sequence s = seq[off[s] ..

@@ -646,14 +646,15 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     __shared__ uint64_t cand[kCand];             // keys of the biggest clusters
     __shared__ uint32_t sh_best;
     __shared__ int32_t sh_ncand;
-    const int n = *list_count;
+    // no list: every strand of the batch, in order (hash_first)
+    const int n = list ? *list_count : 2 * a.rd.nreads;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     for (int i = t; i < kHashSlots; i += THREADS) {
         tkey[i] = kEmpty;
         tval[i] = kFresh;
     }
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const int sidx = list[it];
+        const int sidx = list ? list[it] : it;
         int L;
         if (t == 0) {
             sh_best = 0;
@@ -684,6 +685,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             before += w < wv ? wsum[w] : 0;
             total += wsum[w];
         }
+        if (!list && t == 0) a.out.nhits[sidx] = total;  // the wave tier's count otherwise
         if (total > kHashCap) {
             if (t == 0) next_list[atomicAdd(next_count, 1)] = sidx;
             __syncthreads();
@@ -1065,11 +1067,13 @@ hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildS
 
 hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
     if (a.rd.nreads <= 0) return hipSuccess;
-    hipLaunchKernelGGL(map_strands, dim3(2 * a.rd.nreads), dim3(64), 0, st, a);
+    const bool hash_first = a.hash_first && mid_blocks > 0 && a.ix.cid_ok;
+    if (!hash_first) hipLaunchKernelGGL(map_strands, dim3(2 * a.rd.nreads), dim3(64), 0, st, a);
     if (mid_blocks > 0) {
         if (a.ix.cid_ok)
             hipLaunchKernelGGL((map_strands_hash<512>), dim3(2 * mid_blocks), dim3(512), 0, st,
-                               a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);
+                               a, hash_first ? nullptr : a.ovl_list,
+                               hash_first ? nullptr : a.ovl_count, a.mid_list, a.mid_count);
         else
             hipLaunchKernelGGL((map_strands_wg<256, kMidCap1>), dim3(5 * mid_blocks), dim3(256),
                                0, st, a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);

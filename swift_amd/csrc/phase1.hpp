@@ -90,6 +90,7 @@ struct MapArgs {
                                      // kMidCap1 without 32-bit cluster ids)
     int32_t *big_list, *big_count;   // strands past kMidCap hits
     int mid_blocks;                  // workgroups of map_strands_mid (0: none needed)
+    int hash_first;                  // 1: every strand starts in the hash tier (no wave tier)
     int32_t *error;           // bit 0: a strand exceeded scratch_cap, bit 1: a read
                               // longer than max_len
 };

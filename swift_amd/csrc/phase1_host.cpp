@@ -4,6 +4,7 @@
 // (lookupTable5*, mapHits6GetBestHits, getHash, qryGetReverseComplement,
 // searchQueries3) on top of them.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -270,6 +271,10 @@ int ensure(DevArr &a, size_t bytes) {
 
 // Fills `m` for a launch over `rd`; out arrays may come from the caller
 // (any of them null: the workspace's are used).
+// -1: hash tier first when the index predicts it (mean_hits >= kMapCap in
+// prepare_map); 0: wave tier first; 1: hash tier first
+static std::atomic<int> g_map_mode{-1};
+
 static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len, int max_hits,
                        MapArgs *m, MapWork &w, bool want_strands) {
     if (max_read_len < 0 || max_read_len > kMaxMapRead) {
@@ -328,6 +333,12 @@ static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     m->mid_list = w.mid_list.as<int32_t>();
     m->mid_count = w.flags.as<int32_t>() + 3;
     m->mid_blocks = bound > kMapCap ? 2 * 256 : 0;
+    // Hash tier first when the index's mean bucket size predicts more hits
+    // per strand than the wave tier holds: the wave tier would only count
+    // them and hand them on, and the hash tier counts them again.
+    const double mean_hits = (double)ix.total / (double)std::max<int64_t>(ix.nhash, 1) * nq;
+    const int mode = g_map_mode.load();
+    m->hash_first = mode == 1 || (mode == -1 && mean_hits >= kMapCap);
     return kOk;
 }
 
@@ -737,6 +748,20 @@ void search2Queries(char *qryFile, char *refFile, char *matchFile, int seedLen, 
 }
 
 // ---- status-returning API --------------------------------------------------
+int swift_set_map_mode(int mode) {
+    if (mode < -1 || mode > 1) {
+        set_error("map mode must be -1, 0 or 1");
+        return kErrInvalid;
+    }
+    g_map_mode = mode;
+    return kOk;
+}
+
+int swift_get_map_mode(int *mode) {
+    if (mode) *mode = g_map_mode.load();
+    return kOk;
+}
+
 int swift_index_build(const char *seq, const int64_t *off, const int64_t *len, int nseq,
                       int seed_len, int tuple_ignore_threshold, swift_index **out) {
     if (!out || (nseq > 0 && (!seq || !off || !len))) {

@@ -349,6 +349,18 @@ def search_queries3(qry_path, ref_path, match_path, k=12, max_hits=3, thr=1000,
     return nm.value
 
 
+def set_map_mode(mode: int) -> None:
+    """Mapping tier a strand starts in: -1 by the index (default), 0 the
+    one-wave tier, 1 the hash-cluster tier.  Results are the same."""
+    check(lib().swift_set_map_mode(int(mode)))
+
+
+def get_map_mode() -> int:
+    m = C.c_int()
+    check(lib().swift_get_map_mode(C.byref(m)))
+    return m.value
+
+
 def search2_queries(qry_path, ref_path, match_path, k=12, thr=1000) -> int:
     """search2Queries (src/search2.h:26-27): every read, at most 3 hits per
     strand, shift column written as 0."""

@@ -146,6 +146,42 @@ def test_map_reads_overflow_strands(S, orc):
     check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_map_modes_vs_oracle(S, orc, mode):
+    """Both entry tiers give the oracle's hits: mode 1 sends every strand
+    (a handful of hits, empty reads, thousands of hits) through the
+    hash-cluster tier first, mode 0 through the one-wave tier; the pipeline
+    records match the oracle's too."""
+    from swift_amd import synth
+    rnd = random.Random(90 + mode)
+    seqs = rand_genome(rnd, 4, 3000, 30000, repeats=10)
+    unit = b"ACGTTGCAAC"
+    seqs.append(unit * 700 + bytes(rnd.choice(b"ACGT") for _ in range(3000)))
+    g = np.random.default_rng(90 + mode)
+    reads, _ = synth.sample_reads(g, [np.frombuffer(s, np.uint8) for s in seqs[:4]], 200, 100,
+                                  0.01, 0.001)
+    tail = bytes(rnd.choice(b"ACGT") for _ in range(90))
+    reads += [b"", b"ACGTA", unit * 2 + tail, unit * 6 + tail, unit * 12, seqs[4][-120:]]
+    before = S.get_map_mode()
+    try:
+        S.set_map_mode(mode)
+        assert S.get_map_mode() == mode
+        ix = S.Index.from_sequences(seqs, 10, 100000)
+        got = ix.map_reads(reads, 3)
+        assert (got.nhits > 1024).any() and (got.nhits <= 64).any()
+        oix = orc.Index(seqs, 10, 100000)
+        check_strands(oix, reads, got, 3, orc)
+        recs = ix.align_reads(reads, 3)
+        want = orc.pipeline(oix, reads, 3)
+        assert recs.n == len(want)
+        for i, w in enumerate(want):
+            r = recs.record(i)
+            for gk, wk in zip(REC_KEYS, ORC_KEYS):
+                assert r[gk] == w[wk], (i, gk)
+    finally:
+        S.set_map_mode(before)
+
+
 def test_map_reads_rejects_long_reads(S):
     ix = S.Index.from_sequences([b"ACGT" * 100], 8, 10)
     with pytest.raises(S.SwiftError):
