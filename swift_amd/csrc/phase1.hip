@@ -446,14 +446,17 @@ __device__ int gather_hits_flat(const IndexDev &ix, const unsigned char *qb, int
     return total;
 }
 
-__device__ void load_strand(const MapArgs &a, int sidx, unsigned char *qb, int *len_out) {
-    const int r = sidx >> 1, strand = sidx & 1;
-    int L = a.rd.len[r];
+// Strand sidx of read (sidx >> 1), length L at offset off of the read
+// buffer, into qb (reverse-complemented for the odd strand), and into the
+// strand buffer when there is one.
+__device__ void load_strand_at(const MapArgs &a, int sidx, int L, int64_t off, unsigned char *qb,
+                               int *len_out) {
+    const int strand = sidx & 1;
     if (L < 0 || L > a.max_len) {
         if (threadIdx.x == 0) atomicOr(a.error, 2);
         L = 0;
     }
-    const unsigned char *src = (const unsigned char *)a.rd.seq + a.rd.off[r];
+    const unsigned char *src = (const unsigned char *)a.rd.seq + off;
     char *dst = a.strand_seq ? a.strand_seq + (int64_t)sidx * a.qstride : nullptr;
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
         unsigned char c = src[i];
@@ -463,6 +466,11 @@ __device__ void load_strand(const MapArgs &a, int sidx, unsigned char *qb, int *
         if (dst) dst[at] = (char)v;
     }
     *len_out = L;
+}
+
+__device__ void load_strand(const MapArgs &a, int sidx, unsigned char *qb, int *len_out) {
+    const int r = sidx >> 1;
+    load_strand_at(a, sidx, a.rd.len[r], a.rd.off[r], qb, len_out);
 }
 
 __device__ void emit_best(const MapArgs &a, int sidx, const uint64_t *keys, int n) {
@@ -646,6 +654,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     __shared__ uint64_t cand[kCand];             // keys of the biggest clusters
     __shared__ uint32_t sh_best;
     __shared__ int32_t sh_ncand;
+    __shared__ uint32_t scid0[kMaxIndexSeqs];   // cluster-id base of each sequence
     // no list: every strand of the batch, in order (hash_first)
     const int n = list ? *list_count : 2 * a.rd.nreads;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -653,14 +662,31 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
         tkey[i] = kEmpty;
         tval[i] = kFresh;
     }
+    for (int i = t; i < a.ix.nseq; i += THREADS) scid0[i] = a.ix.cid0[i];
+    // each strand's read length and offset are loaded one strand ahead
+    auto strand_at = [&](int it) { return list ? list[it] : it; };
+    int nx_len = 0;
+    int64_t nx_off = 0;
+    if ((int)blockIdx.x < n) {
+        const int r = strand_at(blockIdx.x) >> 1;
+        nx_len = a.rd.len[r];
+        nx_off = a.rd.off[r];
+    }
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const int sidx = list ? list[it] : it;
+        const int sidx = strand_at(it);
+        const int cur_len = nx_len;
+        const int64_t cur_off = nx_off;
+        if (it + (int)gridDim.x < n) {
+            const int r = strand_at(it + gridDim.x) >> 1;
+            nx_len = a.rd.len[r];
+            nx_off = a.rd.off[r];
+        }
         int L;
         if (t == 0) {
             sh_best = 0;
             sh_ncand = 0;
         }
-        load_strand(a, sidx, qb, &L);
+        load_strand_at(a, sidx, cur_len, cur_off, qb, &L);
         __syncthreads();
         const int nq = L - a.ix.k + 1;
         int c[PER], st[PER], v = 0;
@@ -731,7 +757,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             if (hq[k] < 0) continue;
             const int o = hq[k];
             const uint64_t en = hen[k];
-            const uint32_t cid = a.ix.cid0[(int)(en >> 32)] +
+            const uint32_t cid = scid0[(int)(en >> 32)] +
                                  (uint32_t)((int64_t)(uint32_t)en - o + kShiftBias);
             uint32_t h = cid_slot(cid);
             for (;;) {
@@ -775,14 +801,14 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             int ref = 0;  // last sequence whose cid0 <= cid
             for (int lo = 0, hi = a.ix.nseq - 1; lo <= hi;) {
                 const int mid = (lo + hi) >> 1;
-                if (a.ix.cid0[mid] <= cid) {
+                if (scid0[mid] <= cid) {
                     ref = mid;
                     lo = mid + 1;
                 } else {
                     hi = mid - 1;
                 }
             }
-            const int sh = (int)((int64_t)(cid - a.ix.cid0[ref]) - kShiftBias);
+            const int sh = (int)((int64_t)(cid - scid0[ref]) - kShiftBias);
             a.out.ref[ob + r] = (int8_t)ref;
             a.out.shift[ob + r] = sh;
             a.out.pos[ob + r] = sh + (int)(k & 0xFFFFu);
