@@ -14,7 +14,10 @@ constexpr int kMaxSeeds = 15;          // k <= 15 keeps 4^k buckets < 2^30
 constexpr int kMaxIndexSeqs = 127;     // refIdx is a signed char in the reference
 constexpr int kMaxMapRead = 2047;      // query offsets fit 11 bits
 constexpr int64_t kShiftBias = 2048;   // shift >= -(kMaxMapRead - 1)
-constexpr int kMapCap = 1024;          // hits per strand sorted in LDS by one wave
+constexpr int kMapCap = 256;           // hits per strand sorted in LDS by one wave (2 KB of
+                                       // keys: 8 waves per SIMD fit in LDS)
+constexpr int kHashFirstHits = 1024;   // predicted hits per strand from which strands start
+                                       // in the hash tier
 constexpr int kMidCap1 = 4096;         // ... by one 256-thread workgroup (32 KB LDS)
 constexpr int kHashSlots = 8192;       // cluster table of the hash tier (64 KB LDS)
 constexpr int kHashCap = 6144;         // hits per strand the hash tier counts: table load

@@ -271,8 +271,8 @@ int ensure(DevArr &a, size_t bytes) {
 
 // Fills `m` for a launch over `rd`; out arrays may come from the caller
 // (any of them null: the workspace's are used).
-// -1: hash tier first when the index predicts it (mean_hits >= kMapCap in
-// prepare_map); 0: wave tier first; 1: hash tier first
+// -1: hash tier first when the index predicts it (mean_hits >= kHashFirstHits
+// in prepare_map); 0: wave tier first; 1: hash tier first
 static std::atomic<int> g_map_mode{-1};
 
 static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len, int max_hits,
@@ -338,7 +338,7 @@ static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     // them and hand them on, and the hash tier counts them again.
     const double mean_hits = (double)ix.total / (double)std::max<int64_t>(ix.nhash, 1) * nq;
     const int mode = g_map_mode.load();
-    m->hash_first = mode == 1 || (mode == -1 && mean_hits >= kMapCap);
+    m->hash_first = mode == 1 || (mode == -1 && mean_hits >= kHashFirstHits);
     return kOk;
 }
 

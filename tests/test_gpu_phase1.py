@@ -129,7 +129,7 @@ def test_map_reads_vs_oracle(S, orc, max_hits):
 
 
 def test_map_reads_overflow_strands(S, orc):
-    """Strands past the wave's LDS sort (1024 hits) take the hash-cluster
+    """Strands past the wave's LDS sort (256 hits) take the hash-cluster
     tier up to 6144 hits, the workgroup LDS sort past that, and the global
     scratch path past 16384 hits; every tier in one call."""
     rnd = random.Random(4)
@@ -141,7 +141,7 @@ def test_map_reads_overflow_strands(S, orc):
     ix = S.Index.from_sequences(seqs, 10, 100000)
     got = ix.map_reads(reads, 3)
     h = got.nhits
-    for lo, hi in ((0, 1024), (1024, 4096), (4096, 6144), (6144, 16384), (16384, 1 << 30)):
+    for lo, hi in ((0, 256), (256, 4096), (4096, 6144), (6144, 16384), (16384, 1 << 30)):
         assert ((h > lo) & (h <= hi)).any(), (lo, hi, h)
     check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
 
