@@ -474,6 +474,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.walk != "default" or args.waves >= 0:  # applies to every leg below
+        from swift_amd import align as _align
+        t = _align.get_tuning()
+        _align.set_tuning(args.waves if args.waves >= 0 else t["waves_per_cu"], t["allow_int16"],
+                          t["walk_deferred"] if args.walk == "default"
+                          else args.walk == "deferred")
     if args.c4_only:
         print(json.dumps(c4_leg(args.c4_reads, args.c4_batch, args.c4_parity,
                                 0 if args.no_c5 else args.c5_requests)), flush=True)
@@ -495,11 +501,6 @@ def main():
     from swift_amd import align, synth
     from swift_amd._lib import lib
     from swift_amd.device import synchronize
-    if args.walk != "default" or args.waves >= 0:
-        t = align.get_tuning()
-        align.set_tuning(args.waves if args.waves >= 0 else t["waves_per_cu"], t["allow_int16"],
-                         t["walk_deferred"] if args.walk == "default"
-                         else args.walk == "deferred")
 
     t0 = time.perf_counter()
     qbuf, qoff, qlen, rbuf, roff, rlen = synth.c3_pairs_fast(args.pairs)
