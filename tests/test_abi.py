@@ -33,3 +33,38 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.SwiftError):
         _lib.lib()
+
+
+def test_mode_switches_validate_without_gpu(built):
+    """The process-wide switches (row classes, mapping entry tier, tuning)
+    take their documented ranges, reject others with a message, and touch no
+    device."""
+    L = built.lib()
+    m = C.c_int()
+    assert L.swift_get_map_mode(C.byref(m)) == 0
+    before_map = m.value
+    assert L.swift_get_coarse_rows(C.byref(m)) == 0
+    before_rows = m.value
+    w, i16, wd = C.c_int(), C.c_int(), C.c_int()
+    assert L.swift_get_tuning(C.byref(w), C.byref(i16), C.byref(wd)) == 0
+    before_tuning = (w.value, i16.value, wd.value)
+    try:
+        for mode in (-1, 0, 1):
+            assert L.swift_set_map_mode(mode) == 0
+            assert L.swift_get_map_mode(C.byref(m)) == 0 and m.value == mode
+            assert L.swift_set_coarse_rows(mode) == 0
+            assert L.swift_get_coarse_rows(C.byref(m)) == 0 and m.value == mode
+        for bad in (-2, 2, 99):
+            assert L.swift_set_map_mode(bad) != 0
+            assert b"map mode" in L.swift_last_error()
+            assert L.swift_set_coarse_rows(bad) != 0
+            assert b"row class" in L.swift_last_error()
+        assert L.swift_set_tuning(65, 1, 1) != 0
+        assert L.swift_set_tuning(-1, 1, 1) != 0
+        assert L.swift_set_tuning(8, 1, 0) == 0
+        assert L.swift_get_tuning(C.byref(w), C.byref(i16), C.byref(m)) == 0
+        assert (w.value, i16.value, m.value) == (8, 1, 0)
+    finally:
+        L.swift_set_map_mode(before_map)
+        L.swift_set_coarse_rows(before_rows)
+        L.swift_set_tuning(*before_tuning)
