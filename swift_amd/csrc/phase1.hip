@@ -598,9 +598,10 @@ map_strands_wg(MapArgs a, const int32_t *list, const int32_t *list_count, int32_
 
 // Hash tier (strands of kMapCap < hits <= kHashCap when 32-bit cluster ids
 // fit): no sort.  Each hit is counted into an LDS open-addressing table keyed
-// by its cluster id (count and smallest query offset packed in one word,
-// updated by compare-and-swap); the inserting threads keep each hit's slot
-// and the largest count they produced.  After the biggest count is known,
+// by its cluster id (count in the high half of a word, by atomic add; the
+// smallest query offset in the low half, filled in afterwards for the
+// clusters of the biggest count only); the inserting threads keep each hit's
+// slot and the largest count they produced.  After the biggest count is known,
 // every cluster of that count is listed once, through its hit at the
 // cluster's smallest query offset, and one wave picks the first max_hits of
 // them in cluster-id order, i.e. the (refIdx, shift) order of
@@ -765,13 +766,9 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
                 if (old == kEmpty || old == cid) break;
                 h = (h + 1) & (kHashSlots - 1);
             }
-            uint32_t cur = tval[h], seen, cnt;
-            do {
-                seen = cur;
-                cnt = (seen >> 16) + 1;
-                const uint32_t mo = min(seen & 0xFFFFu, (uint32_t)o);
-                cur = atomicCAS(&tval[h], seen, (cnt << 16) | mo);
-            } while (cur != seen);
+            // count in the high half (the low half stays 0xFFFF until the
+            // offset pass below)
+            const uint32_t cnt = (atomicAdd(&tval[h], 1u << 16) >> 16) + 1;
             mybest = max(mybest, cnt);
             hh[k] = (int)h;
         }
@@ -779,6 +776,13 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
         if (lane == 0) atomicMax(&sh_best, mybest);
         __syncthreads();
         const uint32_t best = sh_best;
+        // smallest query offset of each cluster of the biggest count: every
+        // update of such a slot carries the same count bits
+#pragma unroll
+        for (int k = 0; k < HPT; ++k)
+            if (hq[k] >= 0 && (tval[hh[k]] >> 16) == best)
+                atomicMin(&tval[hh[k]], (best << 16) | (uint32_t)hq[k]);
+        __syncthreads();
         // The clusters of the biggest count, each through its hit at the
         // cluster's smallest query offset (one hit per cluster: two hits of
         // one cluster at the same offset would be the same reference position).
