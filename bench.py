@@ -470,7 +470,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the alignment "
-              "hot path has no cross-GPU exchange; see DESIGN.md for how it shards.",
+              "hot path has no cross-GPU exchange: read batches and pairs shard across "
+              "GPUs as independent processes, each with its own copy of the index.",
               file=sys.stderr)
         sys.exit(2)
 

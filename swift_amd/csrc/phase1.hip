@@ -12,7 +12,7 @@
 //     (refIdx, shift) after sorting by (refIdx, shift, refPos); the first hit
 //     of each biggest cluster is reported, at most maxHits of them, in sorted
 //     order.  The reference draws among more than maxHits tied clusters with
-//     rand() (src/mapHits6.cu:146-160); this build keeps the first maxHits,
+//     rand() (src/mapHits6.cu:146-160); the mapper keeps the first maxHits,
 //     one member of the reference's tie set, and reports the set's size.
 //
 // Layout: the index lives in HBM as an int2 bucket table (start, kept count)
