@@ -5,8 +5,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "sw_host.hpp"
@@ -75,6 +78,14 @@ struct Workspace {
     DevBuf segs;      // seg_task0 / seg_first / seg_len (x2 passes)
     DevBuf counters;  // int32 counters
     DevBuf totals;    // int64 total trace words per pass
+    // Mapped pinned host block the sizes are gathered into (one stream
+    // synchronisation per pass instead of a copy each)
+    struct HostStage {
+        int64_t words;
+        int64_t steps[2 * kRBins];
+        int32_t seg[2 * kRBins + 1];
+        int32_t cnt[16];
+    } *stage = nullptr;
     int cu_count = 0;
     BatchStats stats{};
     // Side streams the row-class kernels are spread over (fork/join by
@@ -140,6 +151,7 @@ void workspace_free(Workspace *w) {
         }
         (void)hipEventDestroy(w->fork);
     }
+    if (w->stage) (void)hipHostFree(w->stage);
     delete w;
 }
 
@@ -293,6 +305,11 @@ static int reserve_common(Workspace &w, int npairs) {
     if ((rc = w.segs.ensure((size_t)2 * 3 * (2 * kRBins + 1) * 4))) return rc;
     if ((rc = w.counters.ensure(256 * 4))) return rc;
     if ((rc = w.totals.ensure((size_t)(kTotSteps + 2 * 2 * kRBins) * 8))) return rc;
+    if (!w.stage) {
+        void *p = nullptr;
+        SWIFT_HIP_TRY(hipHostMalloc(&p, sizeof(Workspace::HostStage), hipHostMallocDefault));
+        w.stage = static_cast<Workspace::HostStage *>(p);
+    }
     return kOk;
 }
 
@@ -324,6 +341,7 @@ int reserve_batch_in(Workspace *w, int npairs, int maxq, int maxr) {
 // counters layout (int32): [4*pass] ntasks, [4*pass+1] binned pairs, then
 // one work-queue cursor per (pass, segment) from kCntQueues on
 constexpr int kCntError = 8;
+constexpr int kCntFlagged = 9;
 constexpr int kCntQueues = 16;
 constexpr int kQueuesPerPass = 2 * kRBins;
 static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStream_t st,
@@ -409,7 +427,7 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
 
     // results start at "score 0": a pair no kernel reaches never walks
     SWIFT_HIP_TRY(launch_fill(w.res.as<int32_t>(), n * (int)(sizeof(PairResult) / 4), 0, st));
-    SWIFT_HIP_TRY(launch_fill(cnt + kCntError, 1, 0, st));
+    SWIFT_HIP_TRY(launch_fill(cnt + kCntError, 2, 0, st));  // error flag, flagged pairs
     SWIFT_HIP_TRY(mark(0, st, w.timed));
     // ---- pass 1: bin every pair
     BinArgs b1;
@@ -419,19 +437,45 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                                  w.task_words.as<int64_t>(), nullptr, tot + 0,
                                  w.scan_sums.as<int64_t>(), mt, st));
     int64_t words1 = 0;
-    int32_t seg1[2 * kRBins + 1], seg2[2 * kRBins + 1], cnt_h[16];
+    int32_t seg1[2 * kRBins + 1], seg2[2 * kRBins + 1];
     int64_t steps1[2 * kRBins], steps2[2 * kRBins];
-    if (sync_sizes) {
-        // trace size, per-class task counts (only non-empty classes are
-        // launched), per-class sweep steps (their cost estimate) and the
-        // error flag in one round trip
-        SWIFT_HIP_TRY(hipMemcpyAsync(&words1, tot + 0, 8, hipMemcpyDeviceToHost, st));
-        SWIFT_HIP_TRY(hipMemcpyAsync(seg1, t1.seg_task0, sizeof(seg1), hipMemcpyDeviceToHost, st));
-        SWIFT_HIP_TRY(hipMemcpyAsync(steps1, tot + kTotSteps, sizeof(steps1),
-                                     hipMemcpyDeviceToHost, st));
-        SWIFT_HIP_TRY(hipMemcpyAsync(cnt_h, cnt, sizeof(cnt_h), hipMemcpyDeviceToHost, st));
+    // Gathers up to four device arrays into the host staging block and waits
+    // for them: one kernel and one synchronisation.
+    using HS = Workspace::HostStage;
+    auto gather = [&](std::initializer_list<std::pair<const void *, int>> spans,
+                      std::initializer_list<int> dst) -> int {
+        Gather g{};
+        for (const auto &sp : spans) {
+            g.src[g.nspans] = static_cast<const uint32_t *>(sp.first);
+            g.nwords[g.nspans++] = sp.second;
+        }
+        int k = 0;
+        for (int d : dst) g.dst[k++] = d / 4;
+        g.out = reinterpret_cast<uint32_t *>(w.stage);
+        SWIFT_HIP_TRY(launch_gather(g, st));
         SWIFT_HIP_TRY(hipStreamSynchronize(st));
-        if (cnt_h[kCntError]) {
+        return kOk;
+    };
+    // trace size, per-class task counts (only non-empty classes are
+    // launched) and per-class sweep steps (their cost estimate)
+    auto read_sizes = [&](const int64_t *words, const int32_t *seg, const int64_t *steps,
+                          const int32_t *c, int nc, int64_t *words_h, int32_t *seg_h,
+                          int64_t *steps_h) -> int {
+        int rc2 = gather({{words, 2}, {seg, 2 * kRBins + 1}, {steps, 4 * kRBins}, {c, nc}},
+                         {(int)offsetof(HS, words), (int)offsetof(HS, seg),
+                          (int)offsetof(HS, steps), (int)offsetof(HS, cnt)});
+        if (rc2) return rc2;
+        *words_h = w.stage->words;
+        std::memcpy(seg_h, w.stage->seg, sizeof(w.stage->seg));
+        std::memcpy(steps_h, w.stage->steps, sizeof(w.stage->steps));
+        return kOk;
+    };
+    if (sync_sizes) {
+        // and the error flag, in the same round trip
+        if ((rc = read_sizes(tot + 0, t1.seg_task0, tot + kTotSteps, cnt, 16, &words1, seg1,
+                             steps1)))
+            return rc;
+        if (w.stage->cnt[kCntError]) {
             set_error("pair lengths beyond maxq/maxr or the batched limits (%d/%d)", kMaxBatchQ,
                       kMaxBatchR);
             return kErrCapacity;
@@ -446,7 +490,7 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                  w.tasks.as<Task>(), nullptr, nullptr, nullptr, w.trace.as<uint32_t>(),
                  w.res.as<PairResult>(), w.ptr.as<PairTrace>(), kc16, h_limit,
                  io.score, io.start, io.end, io.alen, io.qcons, io.rcons, io.cons_off,
-                 tuning().walk_deferred};
+                 tuning().walk_deferred, cnt + kCntFlagged};
     // One persistent kernel per (class, R) segment, each reading its task
     // range from the device (empty segments exit at once), spread over the
     // side streams so the classes run concurrently and one class's tail
@@ -525,7 +569,13 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     SWIFT_HIP_TRY(mark(2, st, w.timed));
     // ---- pass 2: int32 re-run of pairs flagged by the int16 kernel
     int64_t words2 = words1;
-    if (!force32) {
+    int32_t nflagged = 1;
+    if (!force32 && sync_sizes) {
+        // the int16 kernels count what they flag: with none, pass 2 is skipped
+        if ((rc = gather({{cnt + kCntFlagged, 1}}, {(int)offsetof(HS, cnt)}))) return rc;
+        nflagged = w.stage->cnt[0];
+    }
+    if (!force32 && nflagged > 0) {
         BinArgs b2;
         TaskArgs t2;
         pass_args(1, w.res.as<PairResult>(), 1, b2, t2);
@@ -534,12 +584,10 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                                      w.scan_sums.as<int64_t>(), mt, st));
         int32_t nre = 0;
         if (sync_sizes) {
-            SWIFT_HIP_TRY(hipMemcpyAsync(&words2, tot + 1, 8, hipMemcpyDeviceToHost, st));
-            SWIFT_HIP_TRY(hipMemcpyAsync(&nre, cnt + 5, 4, hipMemcpyDeviceToHost, st));
-            SWIFT_HIP_TRY(hipMemcpyAsync(seg2, t2.seg_task0, sizeof(seg2), hipMemcpyDeviceToHost, st));
-            SWIFT_HIP_TRY(hipMemcpyAsync(steps2, tot + kTotSteps + 2 * kRBins, sizeof(steps2),
-                                         hipMemcpyDeviceToHost, st));
-            SWIFT_HIP_TRY(hipStreamSynchronize(st));
+            if ((rc = read_sizes(tot + 1, t2.seg_task0, tot + kTotSteps + 2 * kRBins, cnt + 5, 1,
+                                 &words2, seg2, steps2)))
+                return rc;
+            nre = w.stage->cnt[0];
             w.stats.ntasks32 += seg2[2 * kRBins] - seg2[kRBins];
             w.stats.nrerun = nre;
             if ((size_t)words2 * 4 + 4 > w.trace.bytes) {

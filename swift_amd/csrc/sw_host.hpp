@@ -20,6 +20,16 @@ int plain_rows_for(int qlen);
 int plain_trace_words(int R);
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st);
 
+// up to four device arrays of 32-bit words -> one host staging block
+struct Gather {
+    const uint32_t *src[4];
+    int nwords[4];
+    int dst[4];      // word offset of each array in `out`
+    uint32_t *out;   // mapped pinned host memory
+    int nspans;
+};
+hipError_t launch_gather(const Gather &g, hipStream_t st);
+
 // Device buffer that only grows.
 struct DevBuf {
     void *ptr = nullptr;

@@ -673,8 +673,10 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
             }
         }
         gres[h] = group_best<kBatchG>(bh, brow, bcol, R, PK, a.h_limit, (exotic >> h) & 1);
-        if (g == 0 && pid[h] >= 0)
+        if (g == 0 && pid[h] >= 0) {
             a.res[pid[h]] = PairResult{gres[h].score, gres[h].row, gres[h].col, gres[h].flag};
+            if (gres[h].flag) atomicAdd(a.nflagged, 1);
+        }
     }
 
     const int s = lane;
@@ -1104,6 +1106,14 @@ __global__ void fill_i32(int32_t *p, int n, int32_t v) {
     if (i < n) p[i] = v;
 }
 
+// Copies a few small device arrays into one host staging block (mapped
+// pinned memory), so the host reads them all after a single stream
+// synchronisation instead of one copy each.
+__global__ void __launch_bounds__(256) gather_words(Gather g) {
+    for (int s = 0; s < g.nspans; ++s)
+        for (int i = threadIdx.x; i < g.nwords[s]; i += 256) g.out[g.dst[s] + i] = g.src[s][i];
+}
+
 // ---------------------------------------------------------------------------
 // host-visible launch helpers (used by sw_host.cpp)
 // ---------------------------------------------------------------------------
@@ -1227,6 +1237,11 @@ hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const Gather &g, hipStream_t st) {
+    hipLaunchKernelGGL(gather_words, dim3(1), dim3(256), 0, st, g);
     return hipGetLastError();
 }
 

@@ -85,6 +85,7 @@ struct BatchArgs {
     char *qcons, *rcons;
     const int64_t *coff;
     int walk_deferred;        // 1: record PairTrace descriptors, walk in sw_walk_kernel
+    int32_t *nflagged;        // counts the pairs flagged for the int32 re-run
 };
 
 // outputs of the plain (alignSequences) kernel: reference layout
