@@ -596,7 +596,7 @@ map_strands_wg(MapArgs a, const int32_t *list, const int32_t *list_count, int32_
     }
 }
 
-// Hash tier (strands of kMapCap < hits <= kHashCap when 32-bit cluster ids
+// Hash tier (strands of kMapCap < hits <= CAP when 32-bit cluster ids
 // fit): no sort.  Each hit is counted into an LDS open-addressing table keyed
 // by its cluster id (count in the high half of a word, by atomic add; the
 // smallest query offset in the low half, filled in afterwards for the
@@ -607,8 +607,13 @@ map_strands_wg(MapArgs a, const int32_t *list, const int32_t *list_count, int32_
 // them in cluster-id order, i.e. the (refIdx, shift) order of
 // findBiggestClusters with the first element of each cluster.  The threads
 // then clear only the slots their hits used.
+// The table has SLOTS entries (load <= 3/4 at CAP hits); reads of at most
+// MAXQ bases (the batch's max_len).  A smaller table and k-mer slot array for
+// short reads leave room for more workgroups, i.e. more strands' loads in
+// flight per CU; strands past CAP are handed on to the next tier.
+template <int SLOTS>
 __device__ __forceinline__ uint32_t cid_slot(uint32_t cid) {
-    return (cid * 2654435761u) >> (32 - 13);  // log2(kHashSlots) = 13
+    return (cid * 2654435761u) >> (32 - __builtin_ctz(SLOTS));  // Fibonacci hashing
 }
 
 template <int THREADS>
@@ -633,21 +638,23 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-template <int THREADS>
-__global__ void __launch_bounds__(THREADS)
+template <int THREADS, int SLOTS, int CAP, int MAXQ, int WPE>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int32_t *next_list,
                  int32_t *next_count) {
     constexpr int NW = THREADS / 64;
-    constexpr int PER = (kMaxMapRead + THREADS) / THREADS;
+    constexpr int PER = (MAXQ + THREADS) / THREADS;
     constexpr uint32_t kEmpty = 0xFFFFFFFFu;
     constexpr uint32_t kFresh = 0x0000FFFFu;  // count 0, smallest offset 0xFFFF
     constexpr int NS = THREADS * PER;            // k-mer slots, s = thread * PER + u
-    constexpr int HPT = kHashCap / THREADS;      // hits per thread (a strand here has <= kHashCap)
+    constexpr int HPT = CAP / THREADS;           // hits per thread (a strand here has <= CAP)
     constexpr int kCand = 64;                    // biggest clusters picked by one wave
-    static_assert(kHashCap % THREADS == 0 && kHashCap < 32768, "hit offsets are int16");
-    static_assert(4 * kHashCap <= 3 * kHashSlots, "distinct clusters fill <= 3/4 of the table");
-    __shared__ uint32_t tkey[kHashSlots], tval[kHashSlots];
-    __shared__ unsigned char qb[kMaxMapRead + 1];
+    static_assert(CAP % THREADS == 0 && CAP < 32768, "hit offsets are int16");
+    static_assert(4 * CAP <= 3 * SLOTS && (SLOTS & (SLOTS - 1)) == 0,
+                  "distinct clusters fill <= 3/4 of a power-of-two table");
+    static_assert(MAXQ <= kMaxMapRead, "query offsets fit 11 bits");
+    __shared__ uint32_t tkey[SLOTS], tval[SLOTS];
+    __shared__ unsigned char qb[MAXQ + 1];
     __shared__ int32_t wsum[NW];
     __shared__ uint64_t red[NW];
     __shared__ int16_t sex[NS];                  // block-wide exclusive hit offset of slot s
@@ -659,7 +666,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     // no list: every strand of the batch, in order (hash_first)
     const int n = list ? *list_count : 2 * a.rd.nreads;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    for (int i = t; i < kHashSlots; i += THREADS) {
+    for (int i = t; i < SLOTS; i += THREADS) {
         tkey[i] = kEmpty;
         tval[i] = kFresh;
     }
@@ -713,7 +720,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             total += wsum[w];
         }
         if (!list && t == 0) a.out.nhits[sidx] = total;  // the wave tier's count otherwise
-        if (total > kHashCap) {
+        if (total > CAP) {
             if (t == 0) next_list[atomicAdd(next_count, 1)] = sidx;
             __syncthreads();
             continue;
@@ -760,11 +767,11 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             const uint64_t en = hen[k];
             const uint32_t cid = scid0[(int)(en >> 32)] +
                                  (uint32_t)((int64_t)(uint32_t)en - o + kShiftBias);
-            uint32_t h = cid_slot(cid);
+            uint32_t h = cid_slot<SLOTS>(cid);
             for (;;) {
                 const uint32_t old = atomicCAS(&tkey[h], kEmpty, cid);
                 if (old == kEmpty || old == cid) break;
-                h = (h + 1) & (kHashSlots - 1);
+                h = (h + 1) & (SLOTS - 1);
             }
             // count in the high half (the low half stays 0xFFFF until the
             // offset pass below)
@@ -836,7 +843,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             uint64_t last = 0;
             for (int r = 0; r < take; ++r) {
                 uint64_t mine = ~0ull;
-                for (int i = t; i < kHashSlots; i += THREADS) {
+                for (int i = t; i < SLOTS; i += THREADS) {
                     if ((tval[i] >> 16) != best) continue;
                     const uint64_t k = ((uint64_t)tkey[i] << 16) | (tval[i] & 0xFFFFu);
                     if ((r == 0 || k > last) && k < mine) mine = k;
@@ -1100,11 +1107,23 @@ hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
     const bool hash_first = a.hash_first && mid_blocks > 0 && a.ix.cid_ok;
     if (!hash_first) hipLaunchKernelGGL(map_strands, dim3(2 * a.rd.nreads), dim3(64), 0, st, a);
     if (mid_blocks > 0) {
-        if (a.ix.cid_ok)
-            hipLaunchKernelGGL((map_strands_hash<512>), dim3(2 * mid_blocks), dim3(512), 0, st,
-                               a, hash_first ? nullptr : a.ovl_list,
+        if (a.ix.cid_ok && hash_first && a.max_len <= kShortRead) {
+            // short reads, every strand: first the half-size table (four
+            // workgroups per CU), strands past its cap on to the full one
+            // through the wave tier's (here unused) overflow list
+            constexpr int kWpe = 8;  // waves per SIMD: kWpe / 2 workgroups of 8 waves per CU
+            hipLaunchKernelGGL((map_strands_hash<512, kHashSlots / 2, kHashCap / 2, kShortRead, kWpe>),
+                               dim3(kWpe / 2 * mid_blocks), dim3(512), 0, st, a, nullptr, nullptr,
+                               a.ovl_list, a.ovl_count);
+            hipLaunchKernelGGL((map_strands_hash<512, kHashSlots, kHashCap, kShortRead, 4>),
+                               dim3(2 * mid_blocks), dim3(512), 0, st, a, a.ovl_list, a.ovl_count,
+                               a.mid_list, a.mid_count);
+        } else if (a.ix.cid_ok) {
+            hipLaunchKernelGGL((map_strands_hash<512, kHashSlots, kHashCap, kMaxMapRead, 4>),
+                               dim3(2 * mid_blocks), dim3(512), 0, st, a,
+                               hash_first ? nullptr : a.ovl_list,
                                hash_first ? nullptr : a.ovl_count, a.mid_list, a.mid_count);
-        else
+        } else
             hipLaunchKernelGGL((map_strands_wg<256, kMidCap1>), dim3(5 * mid_blocks), dim3(256),
                                0, st, a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);
         hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), dim3(mid_blocks), dim3(1024), 0, st,

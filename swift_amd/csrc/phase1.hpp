@@ -13,6 +13,8 @@ namespace swift {
 constexpr int kMaxSeeds = 15;          // k <= 15 keeps 4^k buckets < 2^30
 constexpr int kMaxIndexSeqs = 127;     // refIdx is a signed char in the reference
 constexpr int kMaxMapRead = 2047;      // query offsets fit 11 bits
+constexpr int kShortRead = 511;        // batches of reads up to this long take the short-read
+                                       // hash tier kernels
 constexpr int64_t kShiftBias = 2048;   // shift >= -(kMaxMapRead - 1)
 constexpr int kMapCap = 256;           // hits per strand sorted in LDS by one wave (2 KB of
                                        // keys: 8 waves per SIMD fit in LDS)

@@ -128,30 +128,39 @@ def test_map_reads_vs_oracle(S, orc, max_hits):
     check_strands(orc.Index(seqs, 12, 1000), reads, got, max_hits, orc)
 
 
-def test_map_reads_overflow_strands(S, orc):
+@pytest.mark.parametrize("mode", [-1, 1])
+def test_map_reads_overflow_strands(S, orc, mode):
     """Strands past the wave's LDS sort (256 hits) take the hash-cluster
     tier up to 6144 hits, the workgroup LDS sort past that, and the global
-    scratch path past 16384 hits; every tier in one call."""
+    scratch path past 16384 hits; every tier in one call.  Mode 1 starts
+    every strand in the hash tier: for reads this short, its half-size table
+    first (up to 3072 hits), the full one for the strands past that."""
     rnd = random.Random(4)
     unit = b"ACGTTGCAAC"
     seqs = [unit * 2000, bytes(rnd.choice(b"ACGT") for _ in range(20000)) + unit * 300]
     tail = bytes(rnd.choice(b"ACGT") for _ in range(90))
     reads = [unit * 20, (unit * 30)[3:250], seqs[1][100:300], unit[:5] * 40, unit + tail,
              unit * 2 + tail, unit * 5 + tail, unit * 8 + tail]
-    ix = S.Index.from_sequences(seqs, 10, 100000)
-    got = ix.map_reads(reads, 3)
+    before = S.get_map_mode()
+    try:
+        S.set_map_mode(mode)
+        ix = S.Index.from_sequences(seqs, 10, 100000)
+        got = ix.map_reads(reads, 3)
+    finally:
+        S.set_map_mode(before)
     h = got.nhits
-    for lo, hi in ((0, 256), (256, 4096), (4096, 6144), (6144, 16384), (16384, 1 << 30)):
+    for lo, hi in ((0, 256), (256, 3072), (3072, 6144), (6144, 16384), (16384, 1 << 30)):
         assert ((h > lo) & (h <= hi)).any(), (lo, hi, h)
     check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_map_modes_vs_oracle(S, orc, mode):
+@pytest.mark.parametrize("mode,long_read", [(0, False), (1, False), (1, True)])
+def test_map_modes_vs_oracle(S, orc, mode, long_read):
     """Both entry tiers give the oracle's hits: mode 1 sends every strand
     (a handful of hits, empty reads, thousands of hits) through the
     hash-cluster tier first, mode 0 through the one-wave tier; the pipeline
-    records match the oracle's too."""
+    records match the oracle's too.  A read past 511 bases moves the batch
+    from the short-read hash kernels to the general one (mapping only)."""
     from swift_amd import synth
     rnd = random.Random(90 + mode)
     seqs = rand_genome(rnd, 4, 3000, 30000, repeats=10)
@@ -162,6 +171,8 @@ def test_map_modes_vs_oracle(S, orc, mode):
                                   0.01, 0.001)
     tail = bytes(rnd.choice(b"ACGT") for _ in range(90))
     reads += [b"", b"ACGTA", unit * 2 + tail, unit * 6 + tail, unit * 12, seqs[4][-120:]]
+    if long_read:
+        reads.append(seqs[0][1000:1700])
     before = S.get_map_mode()
     try:
         S.set_map_mode(mode)
@@ -171,6 +182,8 @@ def test_map_modes_vs_oracle(S, orc, mode):
         assert (got.nhits > 1024).any() and (got.nhits <= 64).any()
         oix = orc.Index(seqs, 10, 100000)
         check_strands(oix, reads, got, 3, orc)
+        if long_read:
+            return  # the pipeline's windows take reads up to 320 bases
         recs = ix.align_reads(reads, 3)
         want = orc.pipeline(oix, reads, 3)
         assert recs.n == len(want)
