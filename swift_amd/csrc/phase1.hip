@@ -671,33 +671,46 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
         tval[i] = kFresh;
     }
     for (int i = t; i < a.ix.nseq; i += THREADS) scid0[i] = a.ix.cid0[i];
-    // each strand's read length and offset are loaded one strand ahead
+    // Software pipeline over the block's strands: while strand i is counted,
+    // the bases of the block's next strand are loaded (into registers, then
+    // into qb once this strand no longer reads it) and its buckets' heads
+    // gathered, so a strand's entries are its one exposed global round trip.
+    // Read lengths and offsets run one strand further ahead.
     auto strand_at = [&](int it) { return list ? list[it] : it; };
-    int nx_len = 0;
-    int64_t nx_off = 0;
-    if ((int)blockIdx.x < n) {
-        const int r = strand_at(blockIdx.x) >> 1;
-        nx_len = a.rd.len[r];
-        nx_off = a.rd.off[r];
-    }
-    for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const int sidx = strand_at(it);
-        const int cur_len = nx_len;
-        const int64_t cur_off = nx_off;
-        if (it + (int)gridDim.x < n) {
-            const int r = strand_at(it + gridDim.x) >> 1;
-            nx_len = a.rd.len[r];
-            nx_off = a.rd.off[r];
+    const int G = gridDim.x;
+    // a strand's checked length and this thread's bytes of it
+    auto fetch = [&](int len, int64_t off, unsigned char (&nb)[PER]) -> int {
+        if (len < 0 || len > a.max_len) {
+            if (t == 0) atomicOr(a.error, 2);
+            len = 0;
         }
-        int L;
-        if (t == 0) {
-            sh_best = 0;
-            sh_ncand = 0;
+        const unsigned char *src = (const unsigned char *)a.rd.seq + off;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = t + u * THREADS;
+            nb[u] = i < len ? src[i] : 0;
         }
-        load_strand_at(a, sidx, cur_len, cur_off, qb, &L);
-        __syncthreads();
-        const int nq = L - a.ix.k + 1;
-        int c[PER], st[PER], v = 0;
+        return len;
+    };
+    // the strand's bases into qb (reverse complement for strand 1), and into
+    // the pipeline's strand copy
+    auto store = [&](int sidx, int len, const unsigned char (&nb)[PER]) {
+        const int strand = sidx & 1;
+        char *dst = a.strand_seq ? a.strand_seq + (int64_t)sidx * a.qstride : nullptr;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = t + u * THREADS;
+            if (i < len) {
+                const int at = strand ? len - 1 - i : i;
+                const unsigned char b = strand ? rc_byte(nb[u]) : nb[u];
+                qb[at] = b;
+                if (dst) dst[at] = (char)b;
+            }
+        }
+    };
+    // bucket (first entry, count) of each k-mer of the strand in qb
+    auto heads = [&](int len, int (&c)[PER], int (&st)[PER]) {
+        const int nq = len - a.ix.k + 1;
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int o = t + u * THREADS;
@@ -708,8 +721,47 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
                 st[u] = b.x;
                 c[u] = b.y;
             }
-            v += c[u];
         }
+    };
+    int c[PER], st[PER], L0 = 0;
+    int nx_len = 0;
+    int64_t nx_off = 0;
+    if ((int)blockIdx.x < n) {
+        const int s0 = strand_at(blockIdx.x);
+        unsigned char nb[PER];
+        L0 = fetch(a.rd.len[s0 >> 1], a.rd.off[s0 >> 1], nb);
+        store(s0, L0, nb);
+        if ((int)blockIdx.x + G < n) {
+            const int r = strand_at(blockIdx.x + G) >> 1;
+            nx_len = a.rd.len[r];
+            nx_off = a.rd.off[r];
+        }
+    }
+    __syncthreads();
+    heads(L0, c, st);
+    for (int it = blockIdx.x; it < n; it += G) {
+        const int sidx = strand_at(it);
+        const bool more = it + G < n;
+        const int nsidx = more ? strand_at(it + G) : 0;
+        unsigned char nb[PER];
+        int nL = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) nb[u] = 0;
+        if (more) {
+            nL = fetch(nx_len, nx_off, nb);
+            if (it + 2 * G < n) {
+                const int r = strand_at(it + 2 * G) >> 1;
+                nx_len = a.rd.len[r];
+                nx_off = a.rd.off[r];
+            }
+        }
+        if (t == 0) {
+            sh_best = 0;
+            sh_ncand = 0;
+        }
+        int v = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) v += c[u];
         const int incl = wave_incl_scan(v);
         if (lane == 63) wsum[wv] = incl;
         __syncthreads();
@@ -722,7 +774,10 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
         if (!list && t == 0) a.out.nhits[sidx] = total;  // the wave tier's count otherwise
         if (total > CAP) {
             if (t == 0) next_list[atomicAdd(next_count, 1)] = sidx;
+            // qb's last reads (this strand's heads) are behind the barrier above
+            store(nsidx, nL, nb);
             __syncthreads();
+            heads(nL, c, st);
             continue;
         }
         // The strand's hits, flattened over the block: hit e belongs to the
@@ -740,7 +795,7 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
             }
         }
         __syncthreads();
-        uint64_t hen[HPT];
+        uint32_t hen[HPT];
         int hq[HPT], hh[HPT];
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
@@ -753,20 +808,19 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
 #pragma unroll
                 for (int step = NS / 2; step >= 1; step >>= 1)
                     if (sex[sl + step] <= e) sl += step;
-                hen[k] = a.ix.entry[sbase[sl] + e];
+                hen[k] = a.ix.gpos[sbase[sl] + e];
                 hq[k] = sl / PER + (sl % PER) * THREADS;  // the hit's query offset
             }
         }
+        // the next strand's bases (qb is free, see above)
+        store(nsidx, nL, nb);
         // count every hit into the table; remember its slot and the largest
         // count this thread produced
         uint32_t mybest = 0;
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hq[k] < 0) continue;
-            const int o = hq[k];
-            const uint64_t en = hen[k];
-            const uint32_t cid = scid0[(int)(en >> 32)] +
-                                 (uint32_t)((int64_t)(uint32_t)en - o + kShiftBias);
+            const uint32_t cid = hen[k] - (uint32_t)hq[k] + (uint32_t)kShiftBias;
             uint32_t h = cid_slot<SLOTS>(cid);
             for (;;) {
                 const uint32_t old = atomicCAS(&tkey[h], kEmpty, cid);
@@ -782,6 +836,8 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
         mybest = wave_max_u32(mybest);
         if (lane == 0) atomicMax(&sh_best, mybest);
         __syncthreads();
+        // the next strand's heads, in flight through the rest of this one
+        heads(nL, c, st);
         const uint32_t best = sh_best;
         // smallest query offset of each cluster of the biggest count: every
         // update of such a slot carries the same count bits
@@ -1088,6 +1144,18 @@ hipError_t launch_index_build(const IndexDev &ix, int64_t ntup, const IndexBuild
     hipLaunchKernelGGL(idx_scan_blocks, dim3(nblocks), dim3(1024), 0, st, ix, s);
     hipLaunchKernelGGL(idx_scan_sums, dim3(1), dim3(1024), 0, st, s, nblocks);
     hipLaunchKernelGGL(idx_scan_apply, dim3(nblocks), dim3(1024), 0, st, ix, s);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) idx_gpos(IndexDev ix, int64_t total, uint32_t *gpos) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i <= total; i += (int64_t)gridDim.x * 256) {
+        const uint64_t e = ix.entry[i];
+        gpos[i] = i == 0 ? 0xFFFFFFFFu : ix.cid0[(int)(e >> 32)] + (uint32_t)e;
+    }
+}
+
+hipError_t launch_index_gpos(const IndexDev &ix, int64_t total, uint32_t *gpos, hipStream_t st) {
+    hipLaunchKernelGGL(idx_gpos, dim3(tuple_grid(total + 1)), dim3(256), 0, st, ix, total, gpos);
     return hipGetLastError();
 }
 

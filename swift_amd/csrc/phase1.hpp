@@ -49,6 +49,10 @@ struct IndexDev {
     // (refIdx, shift).  cid_ok = 0 when the sum does not fit (> 4 Gbp).
     const uint32_t *cid0;
     int cid_ok;
+    // [total + 1] cid0[refIdx] + refPos of each entry (cid_ok only): a hit's
+    // cluster id is this minus its query offset plus kShiftBias, so the hash
+    // tier reads 4 bytes per hit instead of the 8-byte entry
+    const uint32_t *gpos;
 };
 
 struct IndexBuildScratch {
@@ -116,6 +120,7 @@ struct PairsDev {
 
 hipError_t launch_index_build(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,
                               hipStream_t st);
+hipError_t launch_index_gpos(const IndexDev &ix, int64_t total, uint32_t *gpos, hipStream_t st);
 hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildScratch &s,
                              hipStream_t st);
 // mid_blocks: persistent workgroups of map_strands_mid (0 when no strand can

@@ -233,6 +233,11 @@ static int index_build(const char *bases, const int64_t *off, const int64_t *len
     }
     d = ix->dev();
     SWIFT_HIP_TRY(launch_index_fill(d, ix->ntup, sc, nullptr));
+    if (ix->cid_ok) {
+        // the hash tier's 4-byte form of the (now ordered) entries
+        if ((rc = ix->gpos.alloc(4 * (size_t)(total + 1)))) return rc;
+        SWIFT_HIP_TRY(launch_index_gpos(d, total, ix->gpos.as<uint32_t>(), nullptr));
+    }
     SWIFT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
 }
