@@ -771,33 +771,71 @@ __global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t
     }
 }
 
+// What sw_expand_kernel reads of a pair before expanding it.
+struct ExpandMeta {
+    int p, flags, row, col, len, ql, rl;
+    int64_t co, qo, ro;
+};
+__device__ __forceinline__ ExpandMeta expand_meta(const BatchArgs &a, int p) {
+    const PairResult r = a.res[p];
+    return ExpandMeta{p, r.flags, r.row, r.col, a.alen_out[p], a.qlen[p], a.rlen[p], a.coff[p],
+                      a.qoff[p], a.roff[p]};
+}
+
 __global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32_t *order,
                                                         const int32_t *first_p,
                                                         const int32_t *count,
                                                         const uint32_t *moves, int move_words) {
-    // per wave: the pair's query and reference bytes, then both output strings
+    // per wave: the pair's query and reference bytes (from the aligned dword
+    // holding the first one), then both output strings
     constexpr int kOut = kMaxBatchQ + kMaxBatchR + 4;
-    __shared__ unsigned char lq[4][kMaxBatchQ], lr[4][kMaxBatchR];
+    constexpr int kQW = (kMaxBatchQ + 6) / 4, kRW = (kMaxBatchR + 6) / 4;  // dwords staged
+    __shared__ uint32_t lq4[4][kQW], lr4[4][kRW];
     __shared__ unsigned char oq[4][kOut], orr[4][kOut];
     const int n = *count;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int32_t *ord = order + (first_p ? *first_p : 0);
-    for (int i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
-        const int p = ord[i];
-        const PairResult r = a.res[p];
-        if (r.flags & 1) continue;
-        const int len = a.alen_out[p];
-        const int64_t co = a.coff[p];
-        char *qc = a.qcons + co, *rc = a.rcons + co;
-        const int ql = a.qlen[p], rl = a.rlen[p];
-        const unsigned char *qs = (const unsigned char *)a.qseq + a.qoff[p];
-        const unsigned char *rs = (const unsigned char *)a.rseq + a.roff[p];
+    const int S = gridDim.x * 4;
+    // A pair's order entry is loaded one pair ahead and its metadata during
+    // the previous pair's expansion; its moves and bases are loaded together.
+    int i = blockIdx.x * 4 + wv;
+    ExpandMeta m{};
+    if (i < n) m = expand_meta(a, ord[i]);
+    for (; i < n; i += S) {
+        const int pn = i + S < n ? ord[i + S] : -1;
+        if (m.flags & 1) {  // left to the int32 pass
+            if (pn >= 0) m = expand_meta(a, pn);
+            continue;
+        }
+        const int len = m.len;
+        char *qc = a.qcons + m.co, *rc = a.rcons + m.co;
         // the path only visits rows <= row and columns <= col
-        const int nq = min(ql, r.row), nr = min(rl, r.col);
-        for (int k = lane; k < nq; k += 64) lq[wv][k] = qs[k];
-        for (int k = lane; k < nr; k += 64) lr[wv][k] = rs[k];
+        const int nq = min(m.ql, m.row), nr = min(m.rl, m.col);
         const int nw = (len + 15) >> 4;
-        const uint32_t w = lane < nw ? moves[(int64_t)p * move_words + lane] : 0u;
+        const uint32_t w = lane < nw ? moves[(int64_t)m.p * move_words + lane] : 0u;
+        const uintptr_t qa = (uintptr_t)(a.qseq + m.qo), ra = (uintptr_t)(a.rseq + m.ro);
+        const int qm = (int)(qa & 3), rm = (int)(ra & 3);
+        const uint32_t *qs4 = (const uint32_t *)(qa - qm), *rs4 = (const uint32_t *)(ra - rm);
+        // an aligned dword holding a sequence byte lies within that byte's page
+        // (no dword is read for an empty stretch)
+        const int nqw = nq > 0 ? (qm + nq + 3) >> 2 : 0, nrw = nr > 0 ? (rm + nr + 3) >> 2 : 0;
+        uint32_t vq[(kQW + 63) / 64], vr[(kRW + 63) / 64];
+#pragma unroll
+        for (int u = 0; u < (kQW + 63) / 64; ++u)
+            vq[u] = lane + 64 * u < nqw ? qs4[lane + 64 * u] : 0u;
+#pragma unroll
+        for (int u = 0; u < (kRW + 63) / 64; ++u)
+            vr[u] = lane + 64 * u < nrw ? rs4[lane + 64 * u] : 0u;
+        ExpandMeta mn = m;
+        if (pn >= 0) mn = expand_meta(a, pn);
+#pragma unroll
+        for (int u = 0; u < (kQW + 63) / 64; ++u)
+            if (lane + 64 * u < nqw) lq4[wv][lane + 64 * u] = vq[u];
+#pragma unroll
+        for (int u = 0; u < (kRW + 63) / 64; ++u)
+            if (lane + 64 * u < nrw) lr4[wv][lane + 64 * u] = vr[u];
+        const unsigned char *lq = (const unsigned char *)lq4[wv] + qm;
+        const unsigned char *lr = (const unsigned char *)lr4[wv] + rm;
         const int cnt = lane < nw ? min(16, len - 16 * lane) : 0;
         // row steps: diagonal and up moves; column steps: diagonal and left
         int di = 0, dj = 0;
@@ -816,12 +854,12 @@ __global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32
             }
         }
         wave_lds_sync();  // this wave's lq/lr filled
-        int ci = r.row - (ei - di), cj = r.col - (ej - dj);
+        int ci = m.row - (ei - di), cj = m.col - (ej - dj);
         for (int k = 0; k < cnt; ++k) {
             const int op = (w >> (2 * k)) & 3;
             const int at = 16 * lane + k;
-            oq[wv][at] = op == 2 ? '-' : lq[wv][ci - 1];
-            orr[wv][at] = op == 1 ? '-' : lr[wv][cj - 1];
+            oq[wv][at] = op == 2 ? '-' : lq[ci - 1];
+            orr[wv][at] = op == 1 ? '-' : lr[cj - 1];
             ci -= op != 2;
             cj -= op != 1;
         }
@@ -831,7 +869,7 @@ __global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32
         }
         wave_lds_sync();
         // coalesced copy out (NUL included); dword stores when aligned
-        if ((co & 3) == 0) {
+        if ((m.co & 3) == 0) {
             const int nd = (len + 1 + 3) >> 2;
             for (int k = lane; k < nd; k += 64) {
                 uint32_t vq = 0, vr = 0;
@@ -860,6 +898,7 @@ __global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32
             }
         }
         wave_lds_sync();  // before the next pair overwrites this wave's buffers
+        m = mn;
     }
 }
 
