@@ -246,7 +246,6 @@ def c5_leg(index, seqs, n_requests, seed=20261205):
     every request at once (capacity, reads/s), then Poisson arrivals at 70 %
     of that capacity; latency = submit -> records ready, measured by the
     server."""
-    import threading
     from swift_amd import synth
     from swift_amd.serve import Server
     rng = np.random.default_rng(seed)
@@ -268,6 +267,22 @@ def c5_leg(index, seqs, n_requests, seed=20261205):
                        "lengths U{50..300}, 1.5 % error, index resident",
            "requests": n_requests, "reads": total_reads}
     packed = [Server.pack(r) for r in reqs]
+    # serving runs with the library's default settings, not with what the C3
+    # autotuner chose for one 1M-pair batch (they are process-wide)
+    from swift_amd import align
+    saved = align.get_tuning()
+    align.set_tuning(0, True, True)
+    align.set_coarse_rows(-1)
+    try:
+        return _c5_serve(index, packed, reqs, sizes, total_reads, n_requests, rng, res)
+    finally:
+        align.set_tuning(saved["waves_per_cu"], saved["allow_int16"], saved["walk_deferred"])
+        align.set_coarse_rows(saved["coarse_rows"])
+
+
+def _c5_serve(index, packed, reqs, sizes, total_reads, n_requests, rng, res):
+    import threading
+    from swift_amd.serve import Server
     with Server(index, nslots=4, mem_budget=48 << 30, max_batch_reads=8192) as srv:
         for p in packed[:8]:  # warm the slots
             srv.result(srv.submit_packed(*p), strings=False)
