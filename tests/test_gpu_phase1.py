@@ -154,6 +154,28 @@ def test_map_reads_overflow_strands(S, orc, mode):
     check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
 
 
+def test_map_hash_table_sizes(S, orc):
+    """Hash-first mapping of short reads passes strands on by hit count: up
+    to 3072 the half-size table, up to 6144 the full one, then the workgroup
+    sort.  Strands of 15, 1600, 3200, 4800, 5000 and 6600 hits (counts from
+    the oracle) land in every stage, on both sides of each hand-over."""
+    rnd = random.Random(11)
+    ua, ub = b"ACGTTGCAAC", b"TTGACCGTAG"
+    seqs = [ua * 5000, ub * 1600, bytes(rnd.choice(b"ACGT") for _ in range(20000))]
+    tail = bytes(rnd.choice(b"ACGT") for _ in range(90))
+    reads = [ua + tail, ub * 2 + tail, ub + tail, ua + ub + tail, seqs[2][500:650],
+             ub * 3 + tail]
+    before = S.get_map_mode()
+    try:
+        S.set_map_mode(1)
+        ix = S.Index.from_sequences(seqs, 10, 100000)
+        got = ix.map_reads(reads, 3)
+    finally:
+        S.set_map_mode(before)
+    assert sorted(got.nhits[0::2]) == [15, 1600, 3200, 4800, 5000, 6600]
+    check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
+
+
 @pytest.mark.parametrize("mode,long_read", [(0, False), (1, False), (1, True)])
 def test_map_modes_vs_oracle(S, orc, mode, long_read):
     """Both entry tiers give the oracle's hits: mode 1 sends every strand
