@@ -379,6 +379,37 @@ struct TraceReader {
     }
 };
 
+// TraceReader for the deferred walk kernel, which issues in the gaps the
+// sweeps leave: per step only 32-bit arithmetic on (i, j), the pair's first
+// granule resolved once, rows per word a shift, and the step's word picked
+// by selects, not branches (80 instructions per step against 125).  The
+// fused walk inside the sweep kernels keeps TraceReader: this one's extra
+// live values spill there.
+struct TraceReaderLean {
+    const uint4 *tr;  // the pair's lane group at step 0, word 0
+    int R, W, kl, bitb;
+    uint32_t inv;
+    int cached = -1;
+    uint4 gran;
+    __device__ TraceReaderLean(const uint4 *t, const PairTrace &p)
+        : tr(t + p.base / 4 + p.lane0), R(p.rows), W(p.words), kl((p.half & 2) ? 3 : 4),
+          bitb((p.half & 1) * 16), inv((65536u + (uint32_t)p.rows - 1) / (uint32_t)p.rows) {}
+    __device__ __forceinline__ int code(int i, int j) {
+        const int g = (int)(((uint32_t)(i - 1) * inv) >> 16), k = (i - 1) - g * R;
+        const int t = (j - 1) + g;
+        const int w = k >> kl;  // rows per word: 8 (packed halves) or 16
+        const int bit = bitb + 2 * (k & ((1 << kl) - 1));
+        const int gi = ((t >> 2) * W + w) * 64 + g;
+        if (gi != cached) {
+            gran = tr[gi];
+            cached = gi;
+        }
+        const uint32_t lo = (t & 1) ? gran.y : gran.x, hi = (t & 1) ? gran.w : gran.z;
+        const uint32_t word = (t & 2) ? hi : lo;
+        return (word >> bit) & 3;
+    }
+};
+
 // Word offset of step t, trace word w, lane `lane` in the granule layout.
 template <int W>
 __device__ __forceinline__ int64_t trace_word(int t, int w, int lane) {
@@ -405,13 +436,13 @@ struct WalkOut {
 // in walk order (the reference stores the strings reversed, i.e. in this
 // order) with op 0 = diagonal (both bases), 1 = up (reference gap), 2 = left
 // (query gap).
-template <class Emit>
+template <class Reader = TraceReader, class Emit>
 __device__ __forceinline__ WalkOut walk_cells(const uint32_t *trace, int score, int row, int col,
                                               const PairTrace &pt, int qlen, int rlen, Emit &emit) {
     int start = 0, nrb = -1, tick = 0;
     if (score > 0 && row >= 1 && row <= qlen && col >= 1 && col <= rlen) {
         int ci = row, cj = col, ni, nj;
-        TraceReader rd(reinterpret_cast<const uint4 *>(trace), pt);
+        Reader rd(reinterpret_cast<const uint4 *>(trace), pt);
         auto back = [&](int i, int j, int &oi, int &oj) {
             int c = rd.code(i, j);
             oi = (c & 2) ? i - 1 : i;            // diag (3) and up (2) move up
@@ -762,7 +793,8 @@ __global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t
                 word = 0;
             }
         };
-        WalkOut wo = walk_cells(a.trace, r.score, r.row, r.col, pt, a.qlen[p], a.rlen[p], emit);
+        WalkOut wo = walk_cells<TraceReaderLean>(a.trace, r.score, r.row, r.col, pt, a.qlen[p],
+                                                 a.rlen[p], emit);
         if (wo.len & 15) mv[wo.len >> 4] = word;
         a.score_out[p] = r.score;
         a.start_out[p] = wo.start;
