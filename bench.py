@@ -267,13 +267,12 @@ def c5_leg(index, seqs, n_requests, seed=20261205):
                        "lengths U{50..300}, 1.5 % error, index resident",
            "requests": n_requests, "reads": total_reads}
     packed = [Server.pack(r) for r in reqs]
-    # serving runs with its own settings, not with what the C3 autotuner chose
-    # for one 1M-pair batch (they are process-wide): 8 persistent waves per CU
-    # (at the occupancy limit the p99 of three runs was 6.7-32.6 ms, at 8
-    # waves 5.6-6.6 ms), deferred walk, row classes chosen per batch
+    # serving runs with the library defaults, not with what the C3 autotuner
+    # chose for one 1M-pair batch (the settings are process-wide); the server
+    # keeps its own 8 persistent waves per CU either way
     from swift_amd import align
     saved = align.get_tuning()
-    align.set_tuning(8, True, True)
+    align.set_tuning(0, True, True)
     align.set_coarse_rows(-1)
     try:
         return _c5_serve(index, packed, reqs, sizes, total_reads, n_requests, rng, res)

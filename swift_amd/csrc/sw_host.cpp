@@ -98,6 +98,7 @@ struct Workspace {
     hipEvent_t fork = nullptr, join[kSide] = {};
     bool streams_ready = false;
     bool timed = false;       // records the HIP-event phase marks (the shared workspace)
+    int waves_per_cu = -1;    // persistent waves per CU for this workspace; -1: tuning()'s
     std::mutex mu;            // one batched run at a time per workspace
 
     DevBuf *all_bufs[18] = {&p_trace, &p_res, &p_ptr, &trace, &res, &ptr, &bin_of, &order,
@@ -136,6 +137,8 @@ Workspace *workspace_new(int side_streams) {
     w->nside = std::max(0, std::min(side_streams, (int)Workspace::kSide));
     return w;
 }
+
+void workspace_set_waves(Workspace *w, int waves_per_cu) { w->waves_per_cu = waves_per_cu; }
 
 void workspace_free(Workspace *w) {
     if (!w) return;
@@ -547,7 +550,8 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             b.next_task = cnt + kCntQueues + pass * kQueuesPerPass + seg;
             b.kc = cls == 0 ? kc16 : kc32;
             hipStream_t s2 = nside ? w.side[lane_of[seg]] : st;
-            SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, tuning().waves_per_cu, s2));
+            const int wpc = w.waves_per_cu >= 0 ? w.waves_per_cu : tuning().waves_per_cu;
+            SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, wpc, s2));
             if (ba.walk_deferred && seg_host) {  // per-class walk
                 const int slots = cls == 0 ? kSlots16 : kSlots32;
                 const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;

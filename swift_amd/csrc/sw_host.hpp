@@ -73,6 +73,9 @@ struct Workspace;
 // (0..3); 0 runs them one after another on the caller's stream
 Workspace *workspace_new(int side_streams = 3);
 void workspace_free(Workspace *w);
+// persistent waves per CU of this workspace's row-class kernels, in place of
+// the process-wide tuning (-1: follow it)
+void workspace_set_waves(Workspace *w, int waves_per_cu);
 int64_t workspace_bytes(const Workspace *w);
 int sw_batch_run_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st);
 // Capturable form: no host synchronisation, no allocation (reserve_batch_in
