@@ -53,6 +53,10 @@ int swift_memcpy_htod(void *dst, const void *src, size_t bytes);
 int swift_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int swift_memset_device(void *dst, int value, size_t bytes);
 int swift_device_synchronize(void);
+/* A non-blocking HIP stream (hipStream_t as void*) at the default priority. */
+int swift_stream_create(void **stream);
+int swift_stream_synchronize(void *stream);
+int swift_stream_destroy(void *stream);
 
 /* ---------------------------------------------------- phase 2: plain SW */
 
@@ -75,6 +79,27 @@ void alignSequences(char *querySeq_d, char *refSeq_d, int querySeqLength, int re
                     float *score_d, int *alignStart_d, int *alignEnd_d, int *alignLength_d,
                     char *queryConsensus_d, char *refConsensus_d, int queriesPerIteration,
                     int refsPerIteration, int maxThreadsPerBlock);
+
+/*
+ * Drop-in for smithWaterman_cpu (src/smithWaterman.h:62-66,
+ * src/smithWaterman.cu:44-335): one pair, host buffers.  The caller owns the
+ * (qryLen+1) x (refLen+1) matrices HMatrix, rowBacktracker and
+ * colBacktracker (row-major, index row * (refLen+1) + col) and zeroes them,
+ * as the reference's callers do (src/align.cu:2519-2548).  On return they
+ * hold what the reference leaves there: for rows 1..qryLen and the columns
+ * before the reference's first NUL, the cell's score and the row/column of
+ * the cell its best candidate came from (itself for a zero start); other
+ * cells are not written.  Score, 1-based start/end column, length and the
+ * two consensus strings in walk order (alignLen bytes each, no NUL
+ * written); startPos is left unchanged when the alignment is empty, and
+ * endPos is then startPos - 1.  The pair runs on the GPU; calls from
+ * several host threads run concurrently.  qryLen <= 4096.  Prints and exits
+ * on error.
+ */
+void smithWaterman_cpu(float *HMatrix, int *rowBacktracker, int *colBacktracker, char *refSeq,
+                       int refLen, char *qrySeq, int qryLen, float match, float mismatch,
+                       float gapOpenPenalty, float gapExtPenalty, float *alignScore, int *startPos,
+                       int *endPos, int *alignLen, char *refConsensus, char *qryConsensus);
 
 /* Status-returning form of alignSequences on a given stream (no reference
  * counterpart). */

@@ -81,13 +81,35 @@ void ref_sw_one(const char *ref, int refLen, const char *qry, int qryLen,
     free(q);
 }
 
+/* One pair through smithWaterman_cpu with the caller's matrices
+ * ((qryLen+1) x (refLen+1), zeroed by the caller) and initial *start, so a
+ * test can compare every matrix cell and output the reference leaves. */
+void ref_sw_matrices(const char *ref, int refLen, const char *qry, int qryLen,
+                     float match, float mismatch, float gapOpen, float gapExt,
+                     float *H, int *rowBt, int *colBt, float *score, int *start,
+                     int *end, int *alen, char *refCons, char *qryCons)
+{
+    char *r = (char *) calloc((size_t) refLen + 1, 1);
+    char *q = (char *) calloc((size_t) qryLen + 1, 1);
+    memcpy(r, ref, (size_t) refLen);
+    memcpy(q, qry, (size_t) qryLen);
+    smithWaterman_cpu(H, rowBt, colBt, r, refLen, q, qryLen, match, mismatch,
+                      gapOpen, gapExt, score, start, end, alen, refCons,
+                      qryCons);
+    free(r);
+    free(q);
+}
+
 typedef struct {
     int tid, nthreads, npairs, maxCells;
     const char *qry, *ref;
     const long *qoff, *roff;
     const int *qlen, *rlen;
     float match, mismatch, gapOpen, gapExt;
-    int *score;
+    int *score, *start, *end, *alen;
+    char *qcons, *rcons;
+    const long *consOff;
+    int nstrings;
 } mt_arg;
 
 static void *mt_worker(void *p)
@@ -113,6 +135,15 @@ static void *mt_worker(void *p)
                           a->mismatch, a->gapOpen, a->gapExt, &s, &st, &en,
                           &al, rc, qc);
         a->score[i] = (int) s;
+        if (a->start) {
+            a->start[i] = st;
+            a->end[i] = en;
+            a->alen[i] = al;
+        }
+        if (a->qcons && i < a->nstrings) {
+            memcpy(a->qcons + a->consOff[i], qc, (size_t) (q + r + 2));
+            memcpy(a->rcons + a->consOff[i], rc, (size_t) (q + r + 2));
+        }
     }
     free(H);
     free(rb);
@@ -120,11 +151,16 @@ static void *mt_worker(void *p)
     return NULL;
 }
 
-/* Many pairs on nthreads threads; q + r must stay below 1198. */
-void ref_sw_batch_mt(int npairs, const char *qry, const long *qoff,
-                     const int *qlen, const char *ref, const long *roff,
-                     const int *rlen, float match, float mismatch,
-                     float gapOpen, float gapExt, int *score, int nthreads)
+/* Many pairs on nthreads threads; q + r must stay below 1198.  start, end
+ * and alen (optional, NULL to skip) receive the coordinates; qcons/rcons
+ * (optional) the consensus strings of the first nstrings pairs at
+ * consOff[i] (room for q + r + 2 bytes each). */
+void ref_sw_batch_mt2(int npairs, const char *qry, const long *qoff,
+                      const int *qlen, const char *ref, const long *roff,
+                      const int *rlen, float match, float mismatch,
+                      float gapOpen, float gapExt, int *score, int *start,
+                      int *end, int *alen, char *qcons, char *rcons,
+                      const long *consOff, int nstrings, int nthreads)
 {
     int maxCells = 1;
     for (int i = 0; i < npairs; ++i) {
@@ -140,12 +176,23 @@ void ref_sw_batch_mt(int npairs, const char *qry, const long *qoff,
     mt_arg args[256];
     for (int t = 0; t < nthreads; ++t) {
         mt_arg a = {t, nthreads, npairs, maxCells, qry, ref, qoff, roff,
-                    qlen, rlen, match, mismatch, gapOpen, gapExt, score};
+                    qlen, rlen, match, mismatch, gapOpen, gapExt, score,
+                    start, end, alen, qcons, rcons, consOff, nstrings};
         args[t] = a;
         pthread_create(&th[t], NULL, mt_worker, &args[t]);
     }
     for (int t = 0; t < nthreads; ++t)
         pthread_join(th[t], NULL);
+}
+
+void ref_sw_batch_mt(int npairs, const char *qry, const long *qoff,
+                     const int *qlen, const char *ref, const long *roff,
+                     const int *rlen, float match, float mismatch,
+                     float gapOpen, float gapExt, int *score, int nthreads)
+{
+    ref_sw_batch_mt2(npairs, qry, qoff, qlen, ref, roff, rlen, match, mismatch,
+                     gapOpen, gapExt, score, NULL, NULL, NULL, NULL, NULL,
+                     NULL, 0, nthreads);
 }
 
 int ref_best_hits(int maxHits, char *refIdx, int *shift, int *refPos, int n,

@@ -87,6 +87,33 @@ def align_sequences(queries, refs, query_len=None, ref_len=None, scoring=Scoring
                       [rc[i, :alen[i]].tobytes() for i in range(n)])
 
 
+def smith_waterman_cpu(ref: bytes, qry: bytes, ref_len: int | None = None, scoring=Scoring(),
+                       start_pos: int = 0) -> dict:
+    """The smithWaterman_cpu drop-in (src/smithWaterman.h:62-66) on one
+    pair: zeroed (q+1) x (ref_len+1) matrices in, and out the score, start,
+    end, length, consensus strings (walk order) and the H / rowBacktracker /
+    colBacktracker matrices the call leaves.  start_pos is the caller's
+    initial *startPos (kept when the alignment is empty)."""
+    rl = len(ref) if ref_len is None else ref_len
+    rbuf = C.create_string_buffer(ref[:rl], max(rl, 1) + 1)
+    qbuf = C.create_string_buffer(qry, len(qry) + 1)
+    q = len(qry)
+    H = np.zeros((q + 1, rl + 1), np.float32)
+    rowbt = np.zeros((q + 1, rl + 1), np.int32)
+    colbt = np.zeros((q + 1, rl + 1), np.int32)
+    n = q + rl + 2
+    rc, qc = C.create_string_buffer(n), C.create_string_buffer(n)
+    sc, st, en, ln = C.c_float(), C.c_int(start_pos), C.c_int(), C.c_int()
+    lib().smithWaterman_cpu(
+        H.ctypes.data_as(C.POINTER(C.c_float)), rowbt.ctypes.data_as(C.POINTER(C.c_int)),
+        colbt.ctypes.data_as(C.POINTER(C.c_int)), rbuf, rl, qbuf, q, C.c_float(scoring.match),
+        C.c_float(scoring.mismatch), C.c_float(scoring.gap_open), C.c_float(scoring.gap_ext),
+        C.byref(sc), C.byref(st), C.byref(en), C.byref(ln), rc, qc)
+    return dict(score=sc.value, start=st.value, end=en.value, len=ln.value,
+                ref_cons=rc.raw[:max(ln.value, 0)], qry_cons=qc.raw[:max(ln.value, 0)],
+                H=H, rowbt=rowbt, colbt=colbt, ref_tail=rc.raw[max(ln.value, 0):])
+
+
 class PairBatch:
     """Pairs resident on the device in the batched layout."""
 

@@ -49,6 +49,24 @@ int swift_device_synchronize(void) {
     SWIFT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
 }
+int swift_stream_create(void **stream) {
+    if (!stream) {
+        set_error("null stream pointer");
+        return kErrInvalid;
+    }
+    hipStream_t s = nullptr;
+    SWIFT_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return kOk;
+}
+int swift_stream_synchronize(void *stream) {
+    SWIFT_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return kOk;
+}
+int swift_stream_destroy(void *stream) {
+    SWIFT_HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return kOk;
+}
 
 void alignSequences(char *querySeq_d, char *refSeq_d, int querySeqLength, int refSeqLength,
                     float match, float mismatch, float gapOpenPenalty, float gapExtPenalty,

@@ -511,6 +511,40 @@ int align_reads_in(MapWork &w, Workspace *sws, const IndexObj &ix, const ReadsDe
 // ---------------------------------------------------------------------------
 static IndexObj *g_lt5 = nullptr;
 static int g_lt5_max_hits = 0;
+// Guards g_lt5 and the drop-ins' scratch.  The reference calls
+// lookupTable5MapQry and mapHits6GetBestHits from 16 threads at once
+// (src/search.cu:323-345); here such calls run one after another.  Lock
+// order: g_lt5_mu, then g_p1_mu (taken inside map_reads).
+static std::mutex g_lt5_mu;
+
+// Pinned host block, grow-only: the drop-ins stage their inputs and results
+// through it, one copy each way per call.
+struct PinnedArr {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b) {
+        if (b <= bytes) return kOk;
+        if (ptr) SWIFT_HIP_TRY(hipHostFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        const size_t nb = b + b / 2 + 256;
+        SWIFT_HIP_TRY(hipHostMalloc(&ptr, nb, hipHostMallocDefault));
+        bytes = nb;
+        return kOk;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Scratch of lookupTable5MapQry / mapHits6GetBestHits: grows to the largest
+// call seen and is kept, so a steady stream of calls allocates nothing.
+struct DropinWork {
+    DevArr in, out, kbuf;
+    PinnedArr hin, hout;
+};
+static DropinWork &dropin_work() {
+    static DropinWork *w = new DropinWork;  // lives until the process ends
+    return *w;
+}
 
 static int build_from_fasta(const char *path, int k, int thr, IndexObj **out) {
     RefSeqs r;
@@ -564,11 +598,14 @@ void qryGetReverseComplement(char *seq, int seqLength, char *revComp) {
 
 void lookupTable5Create(const char *refFile, int seedLen, int maxHitsPerQry,
                         int tupleIgnoreThreshold, int *totalTuples) {
-    std::lock_guard<std::mutex> lk(g_p1_mu);
-    delete g_lt5;
-    g_lt5 = nullptr;
+    std::lock_guard<std::mutex> lk(g_lt5_mu);
     IndexObj *ix = nullptr;
-    if (build_from_fasta(refFile, seedLen, tupleIgnoreThreshold, &ix)) die(__func__);
+    {
+        std::lock_guard<std::mutex> lk1(g_p1_mu);
+        delete g_lt5;
+        g_lt5 = nullptr;
+        if (build_from_fasta(refFile, seedLen, tupleIgnoreThreshold, &ix)) die(__func__);
+    }
     g_lt5 = ix;
     g_lt5_max_hits = maxHitsPerQry;
     if (totalTuples) *totalTuples = (int)ix->total;
@@ -576,42 +613,60 @@ void lookupTable5Create(const char *refFile, int seedLen, int maxHitsPerQry,
 
 int lookupTable5MapQry(const Query *query, unsigned int qryLen, unsigned int isRevComp,
                        char *refIdx_bestHits, int *shift_bestHits, int *refPos_bestHits) {
+    std::lock_guard<std::mutex> lk(g_lt5_mu);
     if (!g_lt5) {
         set_error("lookupTable5MapQry before lookupTable5Create");
         die(__func__);
     }
+    if (qryLen > (unsigned)kMaxMapRead) {
+        set_error("reads up to %d bases (got qryLen %u)", kMaxMapRead, qryLen);
+        die(__func__);
+    }
     const char *seq = isRevComp == 1 ? query->revCompSeq : query->seq;
     const int L = (int)qryLen;
-    const int mh = g_lt5_max_hits;
-    DevArr dseq, doff, dlen, dref, dsh, dpos, dn;
-    int64_t off0 = 0;
-    int32_t len0 = L;
-    if (dseq.alloc(std::max(L, 1)) || doff.alloc(8) || dlen.alloc(4) ||
-        dref.alloc(2 * std::max(mh, 1)) || dsh.alloc(8 * std::max(mh, 1)) ||
-        dpos.alloc(8 * std::max(mh, 1)) || dn.alloc(8))
+    const int mh = std::max(g_lt5_max_hits, 0);
+    const size_t stride = (size_t)std::max(mh, 1);
+    DropinWork &w = dropin_work();
+    // in: [offset int64 | length int32 | 4 pad | bases]
+    // out (both strands of the one read; strand 0 is the sequence asked for):
+    // [nbest int32 x 2 | shift int32 x 2 stride | refPos int32 x 2 stride | refIdx int8 x 2 stride]
+    const size_t in_bytes = 16 + (size_t)L, out_bytes = 8 + 18 * stride;
+    if (ensure(w.in, in_bytes + 64) || ensure(w.out, out_bytes) || w.hin.ensure(in_bytes) ||
+        w.hout.ensure(out_bytes))
         die(__func__);
-    if (L > 0 && hipMemcpy(dseq.ptr, seq, L, hipMemcpyHostToDevice) != hipSuccess) die(__func__);
-    if (hipMemcpy(doff.ptr, &off0, 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dlen.ptr, &len0, 4, hipMemcpyHostToDevice) != hipSuccess)
+    char *hin = w.hin.as<char>();
+    const int64_t off0 = 16;
+    memcpy(hin, &off0, 8);
+    memcpy(hin + 8, &L, 4);
+    memset(hin + 12, 0, 4);
+    if (L > 0) memcpy(hin + 16, seq, L);
+    if (hipMemcpyAsync(w.in.ptr, hin, in_bytes, hipMemcpyHostToDevice, nullptr) != hipSuccess)
         die(__func__);
-    ReadsDev rd{dseq.as<char>(), doff.as<int64_t>(), dlen.as<int32_t>(), 1};
-    // strand 0 of this "read" is the requested sequence itself
-    if (map_reads(*g_lt5, rd, std::max(L, 0), mh, dref.as<int8_t>(), dsh.as<int32_t>(),
-                  dpos.as<int32_t>(), dn.as<int32_t>(), nullptr, nullptr, nullptr))
+    char *din = w.in.as<char>(), *dout = w.out.as<char>();
+    ReadsDev rd{din, reinterpret_cast<const int64_t *>(din),
+                reinterpret_cast<const int32_t *>(din + 8), 1};
+    int32_t *d_n = reinterpret_cast<int32_t *>(dout);
+    int32_t *d_sh = d_n + 2, *d_pos = d_sh + 2 * stride;
+    int8_t *d_ref = reinterpret_cast<int8_t *>(d_pos + 2 * stride);
+    if (map_reads(*g_lt5, rd, L, mh, d_ref, d_sh, d_pos, d_n, nullptr, nullptr, nullptr))
         die(__func__);
-    int32_t n[2];
-    if (hipMemcpy(n, dn.ptr, 8, hipMemcpyDeviceToHost) != hipSuccess) die(__func__);
-    if (n[0] > 0) {
-        if (hipMemcpy(refIdx_bestHits, dref.ptr, n[0], hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(shift_bestHits, dsh.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(refPos_bestHits, dpos.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess)
-            die(__func__);
+    char *hout = w.hout.as<char>();
+    if (hipMemcpy(hout, dout, out_bytes, hipMemcpyDeviceToHost) != hipSuccess) die(__func__);
+    const int32_t *h_n = reinterpret_cast<const int32_t *>(hout);
+    const int32_t *h_sh = h_n + 2, *h_pos = h_sh + 2 * stride;
+    const int8_t *h_ref = reinterpret_cast<const int8_t *>(h_pos + 2 * stride);
+    const int n = h_n[0];
+    for (int i = 0; i < n; ++i) {
+        refIdx_bestHits[i] = (char)h_ref[i];
+        shift_bestHits[i] = h_sh[i];
+        refPos_bestHits[i] = h_pos[i];
     }
-    return n[0];
+    return n;
 }
 
 void lookupTable5Delete(void) {
-    std::lock_guard<std::mutex> lk(g_p1_mu);
+    std::lock_guard<std::mutex> lk(g_lt5_mu);
+    std::lock_guard<std::mutex> lk1(g_p1_mu);
     delete g_lt5;
     g_lt5 = nullptr;
 }
@@ -619,34 +674,47 @@ void lookupTable5Delete(void) {
 int mapHits6GetBestHits(int numBestHits, char *refIdx, int *shift, int *refPos, int size,
                         char *refIdx_bestMatch, int *shift_bestMatch, int *refPos_bestMatch) {
     if (size <= 0) return 0;  // src/mapHits6.cu:45-46
+    std::lock_guard<std::mutex> lk(g_lt5_mu);
     const int mh = std::max(numBestHits, 0);
-    DevArr dref, dsh, dpos, oref, osh, opos, on, kbuf;
-    if (dref.alloc(size) || dsh.alloc(4 * (size_t)size) || dpos.alloc(4 * (size_t)size) ||
-        oref.alloc(std::max(mh, 1)) || osh.alloc(4 * std::max(mh, 1)) ||
-        opos.alloc(4 * std::max(mh, 1)) || on.alloc(8) ||
-        kbuf.alloc(16 * (size_t)next_pow2_64(size)))
+    const size_t n = (size_t)size, m = (size_t)std::max(mh, 1);
+    DropinWork &w = dropin_work();
+    // one device block, staged both ways through one pinned block:
+    // [count int32 x 2 | best shift x m | best refPos x m | best refIdx x m | pad to 16 |
+    //  shift x n | refPos x n | refIdx x n]   (the hits are sorted in place)
+    const size_t head = (8 + 9 * m + 15) & ~(size_t)15;
+    const size_t bytes = head + 9 * n;
+    if (ensure(w.out, bytes) || w.hout.ensure(bytes) ||
+        ensure(w.kbuf, 16 * (size_t)next_pow2_64(size)))
         die(__func__);
-    if (hipMemcpy(dref.ptr, refIdx, size, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dsh.ptr, shift, 4 * (size_t)size, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dpos.ptr, refPos, 4 * (size_t)size, hipMemcpyHostToDevice) != hipSuccess)
+    char *h = w.hout.as<char>();
+    memcpy(h + head, shift, 4 * n);
+    memcpy(h + head + 4 * n, refPos, 4 * n);
+    memcpy(h + head + 8 * n, refIdx, n);
+    char *d = w.out.as<char>();
+    if (hipMemcpyAsync(d + head, h + head, 9 * n, hipMemcpyHostToDevice, nullptr) != hipSuccess)
         die(__func__);
-    if (launch_best_hits(dref.as<int8_t>(), dsh.as<int32_t>(), dpos.as<int32_t>(), size, mh,
-                         oref.as<int8_t>(), osh.as<int32_t>(), opos.as<int32_t>(),
-                         on.as<int32_t>(), on.as<int32_t>() + 1, kbuf.ptr, nullptr) != hipSuccess)
+    int32_t *d_n = reinterpret_cast<int32_t *>(d);
+    int32_t *d_osh = d_n + 2, *d_opos = d_osh + m;
+    int8_t *d_oref = reinterpret_cast<int8_t *>(d_opos + m);
+    int32_t *d_sh = reinterpret_cast<int32_t *>(d + head), *d_pos = d_sh + n;
+    int8_t *d_ref = reinterpret_cast<int8_t *>(d_pos + n);
+    if (launch_best_hits(d_ref, d_sh, d_pos, size, mh, d_oref, d_osh, d_opos, d_n, d_n + 1,
+                         w.kbuf.ptr, nullptr) != hipSuccess ||
+        hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
         die(__func__);
-    int32_t n[2];
-    if (hipMemcpy(n, on.ptr, 8, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(refIdx, dref.ptr, size, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(shift, dsh.ptr, 4 * (size_t)size, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(refPos, dpos.ptr, 4 * (size_t)size, hipMemcpyDeviceToHost) != hipSuccess)
-        die(__func__);
-    if (n[0] > 0) {
-        if (hipMemcpy(refIdx_bestMatch, oref.ptr, n[0], hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(shift_bestMatch, osh.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(refPos_bestMatch, opos.ptr, 4 * n[0], hipMemcpyDeviceToHost) != hipSuccess)
-            die(__func__);
+    memcpy(shift, h + head, 4 * n);
+    memcpy(refPos, h + head + 4 * n, 4 * n);
+    memcpy(refIdx, h + head + 8 * n, n);
+    const int32_t *h_n = reinterpret_cast<const int32_t *>(h);
+    const int32_t *h_osh = h_n + 2, *h_opos = h_osh + m;
+    const int8_t *h_oref = reinterpret_cast<const int8_t *>(h_opos + m);
+    const int cnt = h_n[0];
+    for (int i = 0; i < cnt; ++i) {
+        refIdx_bestMatch[i] = (char)h_oref[i];
+        shift_bestMatch[i] = h_osh[i];
+        refPos_bestMatch[i] = h_opos[i];
     }
-    return n[0];
+    return cnt;
 }
 
 // Phase 1 from files to a match file: index refFileName, map both strands of

@@ -72,6 +72,12 @@ const float *batch_timing_ms() { return g_timing.ms; }
 struct Workspace {
     // plain path
     DevBuf p_trace, p_res, p_ptr;
+    // Marks the end of the last plain call's kernels on its stream.  Every
+    // call writes and walks p_trace; a call on another stream waits for this
+    // event first, so kernels of calls on different streams never share the
+    // trace at the same time.
+    hipEvent_t p_done = nullptr;
+    bool p_pending = false;
     // batched path
     DevBuf trace, res, ptr, bin_of, order, order2, tasks, task_words, scan_sums, moves;
     DevBuf bins;      // bin_count, bin_cursor (x2 passes)
@@ -257,6 +263,15 @@ int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match
     // trace words per pair: whole 4-step granules of (rlen + 63) steps
     const int64_t wpp = (int64_t)((rlen + 63 + 3) & ~3) * plain_trace_words(R) * 64;
     int chunk = (int)std::min<int64_t>(npairs, std::max<int64_t>(1, g_plain_budget_words / wpp));
+    if (!w.p_done) SWIFT_HIP_TRY(hipEventCreateWithFlags(&w.p_done, hipEventDisableTiming));
+    if (w.p_pending) {
+        // a larger trace replaces the buffer the last call's kernels may
+        // still use; otherwise this stream queues behind them
+        if ((size_t)chunk * wpp * 4 > w.p_trace.bytes)
+            SWIFT_HIP_TRY(hipEventSynchronize(w.p_done));
+        else
+            SWIFT_HIP_TRY(hipStreamWaitEvent(st, w.p_done, 0));
+    }
     if ((rc = w.p_trace.ensure((size_t)chunk * wpp * 4))) return rc;
     for (int c0 = 0; c0 < npairs; c0 += chunk) {
         int cn = std::min(chunk, npairs - c0);
@@ -264,6 +279,8 @@ int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match
         SWIFT_HIP_TRY(launch_plain(q_d, r_d, qlen, rlen, sc, cn, c0, w.p_trace.as<uint32_t>(),
                                    wpp, out, st));
     }
+    SWIFT_HIP_TRY(hipEventRecord(w.p_done, st));
+    w.p_pending = true;
     return kOk;
 }
 
