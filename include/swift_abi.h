@@ -109,6 +109,25 @@ int swift_align_sequences(char *querySeq_d, char *refSeq_d, int querySeqLength, 
                           char *queryConsensus_d, char *refConsensus_d, int queriesPerIteration,
                           void *stream);
 
+/* swift_align_sequences that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, limits and results as swift_align_sequences;
+ * the traceback scratch is a fixed 64 MiB device array per GPU (held until
+ * the process ends) instead of library-allocated memory, and the pairs run
+ * in chunks that fit it.  SWIFT_ERR_CAPACITY when one pair's traceback
+ * would not fit (not reachable within the plain kernel's length limits).
+ * Every call on one GPU shares that GPU's array: calls whose kernels
+ * overlap on one GPU race, so issue them on one stream or wait for one
+ * before the next.  Launches only; results are complete once the stream is
+ * synchronised. */
+int swift_align_sequences_noalloc(char *querySeq_d, char *refSeq_d, int querySeqLength,
+                                  int refSeqLength, float match, float mismatch,
+                                  float gapOpenPenalty, float gapExtPenalty, float *score_d,
+                                  int *alignStart_d, int *alignEnd_d, int *alignLength_d,
+                                  char *queryConsensus_d, char *refConsensus_d,
+                                  int queriesPerIteration, void *stream);
+
 /* ---------------------------------------------------- phase 2: batched SW
  * No reference counterpart (the reference aligns fixed 100 x 300 tiles
  * only).  npairs query/reference pairs of uneven lengths, all on the device:
