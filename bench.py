@@ -43,38 +43,112 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(qbuf, qoff, qlen, rbuf, roff, rlen, npairs):
+def host_info():
+    """The host the CPU baselines run on: its CPU model, its logical CPU
+    count and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "host_cpus": os.cpu_count() or 1, "allowed_cpus": allowed}
+
+
+# Host threads of the CPU baselines: the reference's own CPU path runs
+# phase 1 on 16 threads and phase 2 on 8 (src/common.h:97, src/align.cu:32);
+# 16 is also the CPU share a one-GPU box gives a process
+CPU_THREADS = 16
+
+
+def cpu_baseline(batch, qbuf, qoff, qlen, rbuf, roff, rlen, npairs, nstrings=20000,
+                 ncalib=100000):
     """The reference's own smithWaterman_cpu (oracle/_ref, compiled from the
-    reference sources) on the first `npairs` pairs, or the oracle port when
-    that library is absent."""
+    reference sources) on the first `npairs` pairs with CPU_THREADS threads.
+    Its results are compared with the batched GPU pass's on the same pairs
+    (scores, coordinates and lengths of every pair, consensus strings of the
+    first `nstrings`), and the oracle port is timed beside it on the first
+    `ncalib` pairs (the port-to-reference factor that scales the port-timed
+    pipeline baselines).  Falls back to the port when oracle/_ref is absent."""
     from oracle import oracle as O
     n = min(npairs, len(qlen))
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, min(CPU_THREADS, host_info()["allowed_cpus"]))
     cells = float((qlen[:n].astype(np.int64) * rlen[:n]).sum())
     R = O.ref()
-    if R is not None:
-        qo = qoff[:n].astype(np.int64).copy()
-        ro = roff[:n].astype(np.int64).copy()
-        ql = qlen[:n].astype(np.int32).copy()
-        rl = rlen[:n].astype(np.int32).copy()
-        sc = np.zeros(n, np.int32)
-        p = lambda a, t: a.ctypes.data_as(C.POINTER(t))
-        t0 = time.perf_counter()
-        R.ref_sw_batch_mt(n, p(qbuf, C.c_char), p(qo, C.c_long), p(ql, C.c_int),
-                          p(rbuf, C.c_char), p(ro, C.c_long), p(rl, C.c_int), C.c_float(2),
-                          C.c_float(-1), C.c_float(-10), C.c_float(-1), p(sc, C.c_int), threads)
-        dt = time.perf_counter() - t0
-        kind = "reference"
-    else:
+    p = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    qo = qoff[:n].astype(np.int64).copy()
+    ro = roff[:n].astype(np.int64).copy()
+    ql = qlen[:n].astype(np.int32).copy()
+    rl = rlen[:n].astype(np.int32).copy()
+    res = {"unit": "GCUPS", "cores": threads, "threads": threads, **host_info()}
+    if R is None:
         qs = [qbuf[qoff[i]:qoff[i] + qlen[i]].tobytes() for i in range(n)]
         rs = [rbuf[roff[i]:roff[i] + rlen[i]].tobytes() for i in range(n)]
         t0 = time.perf_counter()
         O.sw_batch(qs, rs, nthreads=threads)
         dt = time.perf_counter() - t0
-        kind = "port"
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": kind,
-            "sample": f"first {n} pairs of the workload ({cells:.3e} cells), "
-                      f"{threads} threads, {dt:.1f} s"}
+        res.update({"value": round(cells / dt / 1e9, 4), "kind": "port",
+                    "sample": f"first {n} pairs of the workload ({cells:.3e} cells), "
+                              f"{threads} threads, {dt:.1f} s"})
+        return res
+    ns = min(nstrings, n)
+    cstride = ql[:ns].astype(np.int64) + rl[:ns] + 2
+    co = np.zeros(ns, np.int64)
+    co[1:] = np.cumsum(cstride)[:-1]
+    rq = np.zeros(int(cstride.sum()) + 1, np.uint8)
+    rr = np.zeros(int(cstride.sum()) + 1, np.uint8)
+    sc, st, en, al = (np.zeros(n, np.int32) for _ in range(4))
+    t0 = time.perf_counter()
+    R.ref_sw_batch_mt2(n, p(qbuf, C.c_char), p(qo, C.c_long), p(ql, C.c_int),
+                       p(rbuf, C.c_char), p(ro, C.c_long), p(rl, C.c_int), C.c_float(2),
+                       C.c_float(-1), C.c_float(-10), C.c_float(-1), p(sc, C.c_int),
+                       p(st, C.c_int), p(en, C.c_int), p(al, C.c_int), p(rq, C.c_char),
+                       p(rr, C.c_char), p(co, C.c_long), ns, threads)
+    dt = time.perf_counter() - t0
+    res.update({"value": round(cells / dt / 1e9, 4), "kind": "reference",
+                "sample": f"first {n} pairs of the workload ({cells:.3e} cells), "
+                          f"{threads} threads, {dt:.1f} s"})
+    # the GPU pass's results for the same pairs
+    g = batch.results(strings=False)
+    same = (np.array_equal(g.score[:n], sc) and np.array_equal(g.start[:n], st) and
+            np.array_equal(g.end[:n], en) and np.array_equal(g.length[:n], al))
+    gq = batch.qc.download(np.uint8, batch.cons_bytes)
+    gr = batch.rc.download(np.uint8, batch.cons_bytes)
+    sdiff = 0
+    for i in range(ns):
+        a, k, b = int(batch.coff_h[i]), int(al[i]), int(co[i])
+        if gq[a:a + k].tobytes() != rq[b:b + k].tobytes() or \
+                gr[a:a + k].tobytes() != rr[b:b + k].tobytes():
+            sdiff += 1
+    res["parity_vs_reference"] = {
+        "pairs": n, "fields": "score, start, end, length", "identical": bool(same),
+        "string_pairs": ns, "strings_identical": sdiff == 0}
+    # port-to-reference factor on one sample, same threads
+    m = min(ncalib, n)
+    mcells = float((ql[:m].astype(np.int64) * rl[:m]).sum())
+    qs = [qbuf[qoff[i]:qoff[i] + qlen[i]].tobytes() for i in range(m)]
+    rs = [rbuf[roff[i]:roff[i] + rlen[i]].tobytes() for i in range(m)]
+    t0 = time.perf_counter()
+    O.sw_batch(qs, rs, nthreads=threads)
+    tp = time.perf_counter() - t0
+    scm = np.zeros(m, np.int32)
+    t0 = time.perf_counter()
+    R.ref_sw_batch_mt(m, p(qbuf, C.c_char), p(qo, C.c_long), p(ql, C.c_int),
+                      p(rbuf, C.c_char), p(ro, C.c_long), p(rl, C.c_int), C.c_float(2),
+                      C.c_float(-1), C.c_float(-10), C.c_float(-1), p(scm, C.c_int), threads)
+    tr = time.perf_counter() - t0
+    res["calibration"] = {"pairs": m, "threads": threads,
+                          "port_gcups": round(mcells / tp / 1e9, 4),
+                          "reference_gcups": round(mcells / tr / 1e9, 4),
+                          "reference_over_port": round(tp / tr, 4)}
+    return res
 
 
 def plain_pass_setup(qbuf, qoff, qlen, rbuf, roff, rlen):
@@ -112,12 +186,121 @@ def plain_pass(groups):
             C.c_void_p(b["rc"].ptr), n, None))
 
 
-def c2_leg(n_reads, steps, cpu_reads, keep=None):
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+K = 12                       # seed length of every pipeline config
+
+
+def phase1_bytes(read_lens, nhits, nbest):
+    """Algorithmic HBM bytes of phase-1 mapping, per read strand: a bucket
+    start and count per k-mer offset (8 B), refIdx + refPos per hit (5 B), and
+    the best-hit outputs (9 B each)."""
+    lens = np.repeat(np.asarray(read_lens, np.int64), 2)
+    return float((8 * np.maximum(lens - K + 1, 0) + 5 * np.asarray(nhits, np.int64) +
+                  9 * np.asarray(nbest, np.int64)).sum())
+
+
+def record_cells(recs, read_lens, seq_lens):
+    """Smith-Waterman cells of pipeline records: read length x window
+    length, the window being 300 bases from max(0, refPos - 100) clipped at
+    the sequence end (these genomes have no N to trim)."""
+    c = recs.cols
+    if recs.n == 0:
+        return 0.0
+    wl = np.minimum(300, np.asarray(seq_lens, np.int64)[c["ref_idx"]] - c["ref_distance"])
+    return float((np.asarray(read_lens, np.int64)[c["read"]] * np.maximum(wl, 0)).sum())
+
+
+def pipeline_roofline(p1_bytes, cells, secs):
+    """The two-phase pipeline against its roofline: phase 1's algorithmic
+    bytes at the HBM peak plus the SW cells' integer ops at the packed-int16
+    VALU peak; frac = that ideal time / the measured time."""
+    t_hbm = p1_bytes / (HBM_PEAK_GBS * 1e9)
+    t_valu = CELL_OPS * cells / (PEAK_INT16_TOPS * 1e12)
+    return {"bound": "hbm (phase 1) + valu (phase 2)", "phase1_bytes": round(p1_bytes),
+            "sw_cells": round(cells), "ideal_ms": round((t_hbm + t_valu) * 1e3, 4),
+            "measured_ms": round(secs * 1e3, 4), "frac": round((t_hbm + t_valu) / secs, 4),
+            "phase1_hbm_ms": round(t_hbm * 1e3, 4), "sw_valu_ms": round(t_valu * 1e3, 4)}
+
+
+def pipeline_parity(ix, reads, want, mh):
+    """The device pipeline on `reads` against the oracle pipeline's records
+    `want`, record by record, strings included."""
+    got = ix.align_reads(reads, mh)
+    keys = (("read", "read"), ("strand", "strand"), ("ref_idx", "refIdx"), ("shift", "shift"),
+            ("ref_pos", "refPos"), ("nbiggest", "nBiggest"), ("ref_distance", "refDistance"),
+            ("score", "score"), ("start", "start"), ("end", "end"), ("alen", "len"),
+            ("keep", "keep"), ("ntied", "nTied"), ("qry_cons", "qry_cons"),
+            ("ref_cons", "ref_cons"))
+    same = got.n == len(want) and all(
+        got.record(i)[a] == w[b] for i, w in enumerate(want) for a, b in keys)
+    return {"reads": len(reads), "records": len(want), "identical": bool(same)}
+
+
+def oracle_pipeline_baseline(oix, reads, mh, what):
+    """The oracle's CPU pipeline (searchQueries3 -> alignQueries_cpu_mt
+    restated) over `reads` on CPU_THREADS threads, as the reference's CPU
+    path runs on 16 and 8; returns (baseline dict, records)."""
+    from oracle import oracle as O
+    threads = max(1, min(CPU_THREADS, host_info()["allowed_cpus"]))
+    t0 = time.perf_counter()
+    want = O.pipeline_mt(oix, reads, threads, mh)
+    dt = time.perf_counter() - t0
+    return ({"value": round(len(reads) / dt, 1), "unit": "reads/s", "cores": threads,
+             "threads": threads, "kind": "port", **host_info(),
+             "sample": f"{what}: {len(reads)} reads, oracle pipeline on {threads} threads "
+                       "(index build not included); scale by cpu_baseline.calibration."
+                       "reference_over_port for the reference's own speed"}, want)
+
+
+def c1_leg(steps, cpu):
+    """The small config (1 Mbp uniform reference, 1,000 reads of 100 bp at 2 %
+    substitutions and 0.2 % indels, k=12, thr 1000, maxHits 3): reads/s of the
+    whole device pipeline with its roofline fraction, and the oracle's CPU
+    pipeline over the same reads timed beside it and compared record by
+    record."""
+    from swift_amd import search, synth
+    from swift_amd.device import synchronize
+    seqs, reads, _ = synth.c1_workload()
+    t0 = time.perf_counter()
+    ix = search.Index.from_sequences(seqs, 12, 1000)
+    build_s = time.perf_counter() - t0
+    rb = search.ReadBatch(reads)
+    mh = 3
+    out = search.RecordBuffers(2 * rb.n * mh, rb.max_len)
+    ix.align_batch(rb, out, mh)
+    synchronize()
+    reps = max(steps, 1) * 4
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ix.align_batch(rb, out, mh)
+    synchronize()
+    pipe_s = (time.perf_counter() - t0) / reps
+    recs = out.results(strings=False)
+    hits = ix.map_batch(rb, mh)
+    lens = [len(r) for r in reads]
+    res = {"workload": "C1: 1 Mbp uniform reference, 1000 reads of 100 bp (2 % subs, 0.2 % "
+                       "indels), k=12, thr 1000, maxHits 3",
+           "index_build_s": round(build_s, 3),
+           "pipeline_reads_per_s": round(len(reads) / pipe_s, 1),
+           "pipeline_ms": round(pipe_s * 1e3, 3), "records": recs.n,
+           "roofline": pipeline_roofline(phase1_bytes(lens, hits.nhits, hits.nbest),
+                                         record_cells(recs, lens, [len(s) for s in seqs]),
+                                         pipe_s)}
+    if cpu:
+        from oracle import oracle as O
+        oix = O.Index(seqs, 12, 1000)
+        res["cpu_baseline"], want = oracle_pipeline_baseline(oix, reads, mh, "all C1 reads")
+        res["parity_vs_oracle"] = pipeline_parity(ix, reads, want, mh)
+    return res
+
+
+def c2_leg(n_reads, steps, cpu_reads):
     """Phase 1 + phase 2 over the C2 workload: 50 Mbp / 24 sequences, k=12,
     tuple threshold 1000, maxHits 3.  Reports the device index build time,
-    phase-1 reads/s (both strands mapped) with its HBM figure, and reads/s of
-    the whole device pipeline (map -> 300-bp windows -> batched SW -> best-hit
-    choice), with the oracle's CPU pipeline timed on a sample beside it."""
+    phase-1 reads/s (both strands mapped) against the HBM roofline, and
+    reads/s of the whole device pipeline (map -> 300-bp windows -> batched
+    SW -> best-hit choice) against its roofline, with the oracle's CPU
+    pipeline timed and compared on a sample."""
     from swift_amd import search, synth
     from swift_amd._lib import check, lib
     from swift_amd.device import DeviceBuffer, synchronize
@@ -146,10 +329,10 @@ def c2_leg(n_reads, steps, cpu_reads, keep=None):
         map_once()
     synchronize()
     map_s = (time.perf_counter() - t0) / steps
-    hits = nh.download(np.int32, ns).astype(np.int64)
-    best = nb.download(np.int32, ns).astype(np.int64)
-    lens = np.repeat(np.array([len(r) for r in reads], np.int64), 2)
-    p1_bytes = float((8 * np.maximum(lens - 12 + 1, 0) + 5 * hits + 9 * best).sum())
+    hits = nh.download(np.int32, ns)
+    best = nb.download(np.int32, ns)
+    lens = [len(r) for r in reads]
+    p1 = phase1_bytes(lens, hits, best)
     out = search.RecordBuffers(2 * rb.n * mh, rb.max_len)
     ix.align_batch(rb, out, mh)
     synchronize()
@@ -159,114 +342,49 @@ def c2_leg(n_reads, steps, cpu_reads, keep=None):
     synchronize()
     pipe_s = (time.perf_counter() - t0) / steps
     recs = out.results(strings=False)
-    if keep is not None:
-        keep["index"], keep["seqs"] = ix, seqs
     res = {"workload": "C2: 50 Mbp / 24 sequences (GC 41 %, 5 % tandem repeats), "
                        f"{len(reads)} reads of 100 bp (5 % random), k=12, thr 1000, maxHits 3",
            "index_build_s": round(build_s, 3), "kept_tuples": info["total"],
            "phase1_reads_per_s": round(len(reads) / map_s, 1),
            "phase1_ms": round(map_s * 1e3, 3),
-           "phase1_hbm_gbs": round(p1_bytes / map_s / 1e9, 2),
+           "phase1_roofline": {"bound": "hbm", "achieved": round(p1 / map_s / 1e9, 2),
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(p1 / map_s / 1e9 / HBM_PEAK_GBS, 5),
+                               "bytes": round(p1)},
            "pipeline_reads_per_s": round(len(reads) / pipe_s, 1),
            "pipeline_ms": round(pipe_s * 1e3, 3), "records": recs.n,
+           "pipeline_roofline": pipeline_roofline(p1, record_cells(recs, lens,
+                                                                  [len(x) for x in seqs]),
+                                                  pipe_s),
            "hits_per_strand": round(float(hits.mean()), 2)}
     if cpu_reads > 0:
         from oracle import oracle as O
-        sample = reads[:cpu_reads]
-        t0 = time.perf_counter()
+        # the first reads and the fully random 5 % at the end
+        nrand = min(cpu_reads // 20, len(reads) - n_reads)
+        sample = reads[:cpu_reads - nrand] + reads[len(reads) - nrand:]
         oix = O.Index(seqs, 12, 1000)
-        t1 = time.perf_counter()
-        want = O.pipeline(oix, sample, mh)
-        dt = time.perf_counter() - t1
-        res["cpu_baseline"] = {"value": round(len(sample) / dt, 1), "unit": "reads/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"first {len(sample)} reads, oracle pipeline "
-                                         f"(index build {t1 - t0:.1f} s not included)"}
+        res["cpu_baseline"], want = oracle_pipeline_baseline(
+            oix, sample, mh, f"first {cpu_reads - nrand} C2 reads + {nrand} random reads")
         res["parity_vs_oracle"] = pipeline_parity(ix, sample, want, mh)
     return res
 
 
-def pipeline_parity(ix, reads, want, mh):
-    """The device pipeline on `reads` against the oracle pipeline's records
-    `want`, record by record, strings included."""
-    got = ix.align_reads(reads, mh)
-    keys = (("read", "read"), ("strand", "strand"), ("ref_idx", "refIdx"),
-            ("ref_pos", "refPos"), ("score", "score"), ("start", "start"), ("end", "end"),
-            ("alen", "len"), ("keep", "keep"), ("qry_cons", "qry_cons"),
-            ("ref_cons", "ref_cons"))
-    same = got.n == len(want) and all(
-        got.record(i)[a] == w[b] for i, w in enumerate(want) for a, b in keys)
-    return {"reads": len(reads), "records": len(want), "identical": bool(same)}
-
-
-def c1_leg(steps, cpu):
-    """The small config (1 Mbp uniform reference, 1,000 reads of 100 bp at 2 %
-    substitutions and 0.2 % indels, k=12, thr 1000, maxHits 3): reads/s of the
-    whole device pipeline, and the oracle's CPU pipeline over the same reads
-    timed beside it and compared record by record."""
-    from swift_amd import search, synth
-    from swift_amd.device import synchronize
-    seqs, reads, _ = synth.c1_workload()
-    t0 = time.perf_counter()
-    ix = search.Index.from_sequences(seqs, 12, 1000)
-    build_s = time.perf_counter() - t0
-    rb = search.ReadBatch(reads)
-    mh = 3
-    out = search.RecordBuffers(2 * rb.n * mh, rb.max_len)
-    ix.align_batch(rb, out, mh)
-    synchronize()
-    t0 = time.perf_counter()
-    for _ in range(max(steps, 1) * 4):
-        ix.align_batch(rb, out, mh)
-    synchronize()
-    pipe_s = (time.perf_counter() - t0) / (max(steps, 1) * 4)
-    res = {"workload": "C1: 1 Mbp uniform reference, 1000 reads of 100 bp (2 % subs, 0.2 % "
-                       "indels), k=12, thr 1000, maxHits 3",
-           "index_build_s": round(build_s, 3),
-           "pipeline_reads_per_s": round(len(reads) / pipe_s, 1),
-           "pipeline_ms": round(pipe_s * 1e3, 3), "records": out.results(strings=False).n}
-    if cpu:
-        from oracle import oracle as O
-        t0 = time.perf_counter()
-        oix = O.Index(seqs, 12, 1000)
-        t1 = time.perf_counter()
-        want = O.pipeline(oix, reads, mh)
-        dt = time.perf_counter() - t1
-        res["cpu_baseline"] = {"value": round(len(reads) / dt, 1), "unit": "reads/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"all {len(reads)} reads, oracle pipeline "
-                                         f"(index build {t1 - t0:.2f} s not included)"}
-        res["parity_vs_oracle"] = pipeline_parity(ix, reads, want, mh)
-    return res
-
-
-def c5_leg(index, seqs, n_requests, seed=20261205):
-    """Serving: read batches of log-uniform size 1..4096 (lengths U{50..300},
-    1.5 % error) submitted to a swift_server with the index resident.  First
-    every request at once (capacity, reads/s), then Poisson arrivals at 70 %
-    of that capacity; latency = submit -> records ready, measured by the
+def c5_leg(index, g, off, lens, n_requests, seed=20261205, p1_per_kmer_read=None):
+    """Serving: requests of log-uniform 1..4096 reads (lengths U{50..300},
+    1.5 % substitutions, every read drawn afresh; synth.c5_requests)
+    submitted to a swift_server with the index resident.  First every
+    request at once (capacity, reads/s), then Poisson arrivals at 70 % of
+    that capacity; latency = submit -> records ready, measured by the
     server."""
     from swift_amd import synth
-    from swift_amd.serve import Server
     rng = np.random.default_rng(seed)
-    arr = [np.frombuffer(s, np.uint8) if isinstance(s, bytes) else s for s in seqs]
-    pool = []
-    for L in range(50, 301, 10):
-        rr, _ = synth.sample_reads(rng, arr, 600, L, 0.015, 0.0015)
-        pool += rr
-    order = rng.permutation(len(pool))
-    pool = [pool[i] for i in order]
-    sizes = np.exp(rng.uniform(0, np.log(4096), n_requests)).astype(int)
-    reqs = []
-    for n in sizes:
-        o = int(rng.integers(0, len(pool) - n))
-        reqs.append(pool[o:o + n])
+    packed = synth.c5_requests(rng, g, off, lens, n_requests)
+    sizes = np.array([len(p[2]) for p in packed])
     total_reads = int(sizes.sum())
-    log(f"[bench] C5: {n_requests} requests, {total_reads} reads (pool {len(pool)})")
+    log(f"[bench] C5: {n_requests} requests, {total_reads} reads")
     res = {"workload": f"C5 serving: {n_requests} requests of log-uniform 1..4096 reads, "
-                       "lengths U{50..300}, 1.5 % error, index resident",
+                       "lengths U{50..300}, 1.5 % substitutions, index resident",
            "requests": n_requests, "reads": total_reads}
-    packed = [Server.pack(r) for r in reqs]
     # serving runs with the library defaults, not with what the C3 autotuner
     # chose for one 1M-pair batch (the settings are process-wide); the server
     # keeps its own 8 persistent waves per CU either way
@@ -275,13 +393,22 @@ def c5_leg(index, seqs, n_requests, seed=20261205):
     align.set_tuning(0, True, True)
     align.set_coarse_rows(-1)
     try:
-        return _c5_serve(index, packed, reqs, sizes, total_reads, n_requests, rng, res)
+        res = _c5_serve(index, packed, sizes, total_reads, n_requests, rng, res, lens)
     finally:
         align.set_tuning(saved["waves_per_cu"], saved["allow_int16"], saved["walk_deferred"])
         align.set_coarse_rows(saved["coarse_rows"])
+    if p1_per_kmer_read is not None:
+        # phase-1 bytes per read scale with its k-mer count (L - k + 1)
+        kmers = float(sum(np.maximum(p[2].astype(np.int64) - K + 1, 0).sum() for p in packed))
+        rf = pipeline_roofline(p1_per_kmer_read * kmers, res.pop("capacity_cells"),
+                               res["capacity_s"])
+        rf["note"] = ("capacity run (every request at once); SW cells from the served "
+                      "records, phase-1 bytes per k-mer from the C4 stream")
+        res["capacity_roofline"] = rf
+    return res
 
 
-def _c5_serve(index, packed, reqs, sizes, total_reads, n_requests, rng, res):
+def _c5_serve(index, packed, sizes, total_reads, n_requests, rng, res, seq_lens):
     import threading
     from swift_amd.serve import Server
     with Server(index, nslots=4, mem_budget=48 << 30, max_batch_reads=8192) as srv:
@@ -289,9 +416,10 @@ def _c5_serve(index, packed, reqs, sizes, total_reads, n_requests, rng, res):
             srv.result(srv.submit_packed(*p), strings=False)
         t0 = time.perf_counter()
         ids = [srv.submit_packed(*p) for p in packed]
-        for i in ids:
-            srv.result(i, strings=False)
+        recs = [srv.result(i, strings=False)[0] for i in ids]
         cap_s = time.perf_counter() - t0
+        res["capacity_cells"] = sum(record_cells(r, p[2], seq_lens)
+                                    for r, p in zip(recs, packed))
         capacity = total_reads / cap_s
         lam = 0.7 * capacity
         mean_size = total_reads / n_requests
@@ -322,6 +450,7 @@ def _c5_serve(index, packed, reqs, sizes, total_reads, n_requests, rng, res):
         st = srv.stats()
     lat = np.array(lat)
     res.update({"capacity_reads_per_s": round(capacity, 1),
+                "capacity_s": round(cap_s, 4),
                 "arrival_rate_reads_per_s": round(lam, 1),
                 "arrival_rate_requests_per_s": round(rate, 2),
                 "served_reads_per_s": round(total_reads / run_s, 1),
@@ -346,7 +475,6 @@ def c4_graph(ix, g, off, lens, rng, n_reads, batch):
     while done < n_reads:
         nb = min(batch, n_reads - done)
         buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, nb, 150, 0.01)
-        assert np.array_equal(ro, np.arange(nb, dtype=np.int64) * 150)
         slot_len = np.zeros(batch, np.int32)
         slot_len[:nb] = rl
         synchronize()
@@ -359,23 +487,26 @@ def c4_graph(ix, g, off, lens, rng, n_reads, batch):
         check(lib().swift_pipeline_graph_check(pg._h))
         nrec += int(pg.out.bufs["nrecords"].download(np.int32, 1)[0])
         done += nb
-        log(f"[bench] C4 graph {done}/{n_reads} reads, {dt:.2f} s on the device path")
+        if done % (8 * batch) == 0 or done == n_reads:
+            log(f"[bench] C4 graph {done}/{n_reads} reads, {dt:.2f} s on the device path")
     buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, 20000, 150, 0.01)
     reads = [buf[o:o + 150].tobytes() for o in ro]
     a = pg.run(reads)
     b = ix.align_reads(reads, 3)
     same = a.n == b.n and all(a.record(i) == b.record(i) for i in range(a.n))
     pg.close()
-    return {"reads_per_s": round(n_reads / dt, 1), "records": nrec,
+    return {"reads_per_s": round(n_reads / dt, 1), "records": nrec, "seconds": round(dt, 3),
             "vs_eager": {"reads": len(reads), "records": a.n, "identical": bool(same)}}
 
 
 def c4_leg(n_reads, batch, parity_reads, c5_requests=0, graph=True):
     """Genome scale: 3.1 Gbp / 25 sequences (GC 41 %, 45 % interspersed
-    repeats), k=12 index resident in HBM, 10M reads of 150 bp at 1 %
-    substitutions streamed through the device pipeline in batches.  Reports
-    index build time and reads/s of the timed batches (upload of each batch
-    included); optionally checks `parity_reads` reads against the oracle."""
+    repeats), k=12 index resident in HBM, `n_reads` reads of 150 bp at 1 %
+    substitutions streamed through the device pipeline in batches (upload of
+    each batch included), eager and as hipGraph replays, each with its
+    roofline fraction; `parity_reads` reads compared with the oracle
+    pipeline over the same genome; then the C5 serving leg on this resident
+    index."""
     from swift_amd import search, synth
     from swift_amd.device import synchronize
     t0 = time.perf_counter()
@@ -387,14 +518,23 @@ def c4_leg(n_reads, batch, parity_reads, c5_requests=0, graph=True):
     info = ix.info()
     log(f"[bench] C4 index: {info} in {build_s:.2f} s")
     rng = np.random.default_rng(20261204)
-    buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, min(batch, n_reads), 150, 0.01)
-    hits = ix.map_batch(search.ReadBatch.from_packed(buf, ro, rl, 150), 3).nhits
+    nb0 = min(batch, n_reads)
+    buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, nb0, 150, 0.01)
+    rb0 = search.ReadBatch.from_packed(buf, ro, rl, 150)
+    sh = ix.map_batch(rb0, 3)
+    hits = sh.nhits
     hit_stats = {"mean": round(float(hits.mean()), 1), "p50": int(np.percentile(hits, 50)),
                  "p99": int(np.percentile(hits, 99)), "max": int(hits.max()),
                  "frac_over_1024": round(float((hits > 1024).mean()), 4),
                  "frac_over_16384": round(float((hits > 16384).mean()), 5)}
     log(f"[bench] C4 hits per strand: {hit_stats}")
+    # per-read algorithmic phase-1 bytes and per-record SW cells, from this
+    # first batch (the stream's reads are drawn alike)
+    p1_per_read = phase1_bytes(rl, hits, sh.nbest) / nb0
     out = search.RecordBuffers(2 * batch * 3, 150)
+    ix.align_batch(rb0, out, 3)
+    r0 = out.results(strings=False)
+    cells_per_rec = record_cells(r0, rl, lens) / max(r0.n, 1)
     dt, nrec, done = 0.0, 0, 0
     while done < n_reads:
         nb = min(batch, n_reads - done)
@@ -406,34 +546,37 @@ def c4_leg(n_reads, batch, parity_reads, c5_requests=0, graph=True):
         dt += time.perf_counter() - t0
         nrec += int(out.bufs["nrecords"].download(np.int32, 1)[0])
         done += nb
-        log(f"[bench] C4 {done}/{n_reads} reads, {dt:.2f} s on the device path")
+        if done % (8 * batch) == 0 or done == n_reads:
+            log(f"[bench] C4 {done}/{n_reads} reads, {dt:.2f} s on the device path")
     res = {"workload": "C4: 3.1 Gbp / 25 sequences (GC 41 %, 45 % repeats of 300-6000 bp at "
                        f"10-20 % divergence), {n_reads} reads of 150 bp, 1 % substitutions, "
                        "k=12, thr 1000, maxHits 3",
            "index_build_s": round(build_s, 2), "kept_tuples": info["total"],
            "largest_bucket": info["max_bucket"], "reads_per_s": round(n_reads / dt, 1),
-           "records": nrec, "batch_reads": batch, "hits_per_strand": hit_stats}
+           "seconds": round(dt, 3), "records": nrec, "batch_reads": batch,
+           "hits_per_strand": hit_stats,
+           "roofline": pipeline_roofline(p1_per_read * n_reads, cells_per_rec * nrec, dt)}
+    res["roofline"]["note"] = ("phase-1 bytes per read and SW cells per record from the "
+                               "first batch, scaled to the stream")
     if graph:
         res["graph"] = c4_graph(ix, g, off, lens, rng, n_reads, batch)
+        res["graph"]["roofline"] = pipeline_roofline(
+            p1_per_read * n_reads, cells_per_rec * res["graph"]["records"],
+            res["graph"]["seconds"])
     if parity_reads > 0:
         from oracle import oracle as O
         buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, parity_reads, 150, 0.01)
         reads = [buf[o:o + 150].tobytes() for o in ro]
-        seqs = [g[o:o + n].tobytes() for o, n in zip(off, lens)]
         t0 = time.perf_counter()
-        want = O.pipeline(O.Index(seqs, 12, 1000), reads, 3)
-        got = ix.align_reads(reads, 3)
-        keys = (("read", "read"), ("strand", "strand"), ("ref_idx", "refIdx"),
-                ("ref_pos", "refPos"), ("score", "score"), ("start", "start"), ("end", "end"),
-                ("alen", "len"), ("keep", "keep"), ("qry_cons", "qry_cons"),
-                ("ref_cons", "ref_cons"))
-        same = got.n == len(want) and all(
-            got.record(i)[a] == w[b] for i, w in enumerate(want) for a, b in keys)
-        res["parity_vs_oracle"] = {"reads": parity_reads, "records": len(want),
-                                   "identical": bool(same),
-                                   "oracle_s": round(time.perf_counter() - t0, 1)}
+        oix = O.Index([g[o:o + n].tobytes() for o, n in zip(off, lens)], 12, 1000)
+        log(f"[bench] C4 oracle index in {time.perf_counter() - t0:.1f} s")
+        base, want = oracle_pipeline_baseline(oix, reads, 3, "C4 reads")
+        res["cpu_baseline"] = base
+        res["parity_vs_oracle"] = pipeline_parity(ix, reads, want, 3)
+        del oix
     if c5_requests > 0:
-        res["c5"] = c5_leg(ix, [g[o:o + n] for o, n in zip(off, lens)], c5_requests)
+        res["c5"] = c5_leg(ix, g, off, lens, c5_requests,
+                           p1_per_kmer_read=p1_per_read / (150 - K + 1))
         res["c5"]["index"] = "C4 index (3.1 Gbp / 25 sequences, k=12) resident"
     return res
 
@@ -449,17 +592,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pairs", type=int, default=800000)
     ap.add_argument("--c2-reads", type=int, default=100000)
-    ap.add_argument("--c2-cpu-reads", type=int, default=3000)
+    ap.add_argument("--c2-cpu-reads", type=int, default=16000,
+                    help="C2 reads (95 %% mapped, 5 %% random) the oracle pipeline times and "
+                         "checks")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the C4 hipGraph stream")
     ap.add_argument("--c5-requests", type=int, default=400)
-    ap.add_argument("--c4-only", action="store_true", help="only the genome-scale C4 leg")
+    ap.add_argument("--c4-only", action="store_true", help="only the genome-scale C4 leg "
+                    "(and C5 on its index)")
     ap.add_argument("--c4-reads", type=int, default=10_000_000)
     ap.add_argument("--c4-batch", type=int, default=250_000)
-    ap.add_argument("--c4-parity", type=int, default=0, metavar="READS")
-    ap.add_argument("--c5-only", action="store_true", help="only the C5 serving leg on the C2 "
-                    "index (for profiling)")
+    ap.add_argument("--c4-parity", type=int, default=300, metavar="READS")
     ap.add_argument("--c2-only", action="store_true", help="only the C2 pipeline leg (no JSON "
                     "bench line contract; for profiling)")
     ap.add_argument("--autotune", type=int, default=-1, metavar="PAIRS",
@@ -498,21 +644,14 @@ def main():
                           t["walk_deferred"] if args.walk == "default"
                           else args.walk == "deferred")
     if args.c4_only:
-        print(json.dumps(c4_leg(args.c4_reads, args.c4_batch, args.c4_parity,
-                                0 if args.no_c5 else args.c5_requests)), flush=True)
-        return
-    if args.c5_only:
-        from swift_amd import search, synth
-        seqs, _, _ = synth.c2_workload(n_reads=10)
-        ix = search.Index.from_sequences(seqs, 12, 1000)
-        print(json.dumps(c5_leg(ix, seqs, args.c5_requests)), flush=True)
+        print(json.dumps(c4_leg(args.c4_reads, args.c4_batch,
+                                0 if args.no_cpu else args.c4_parity,
+                                0 if args.no_c5 else args.c5_requests, not args.no_graph)),
+              flush=True)
         return
     if args.c2_only:
-        keep = {}
-        res = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads, keep)
-        if not args.no_c5:
-            res["c5"] = c5_leg(keep["index"], keep["seqs"], args.c5_requests)
-        print(json.dumps(res), flush=True)
+        print(json.dumps(c2_leg(args.c2_reads, args.steps,
+                                0 if args.no_cpu else args.c2_cpu_reads)), flush=True)
         return
 
     from swift_amd import align, synth
@@ -606,22 +745,26 @@ def main():
                                  "calls": len(groups)}
         out["speedup_vs_plain"] = round(gcups / (cells / pdt / 1e9), 3)
         del groups
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(batch, qbuf, qoff, qlen, rbuf, roff, rlen,
+                                           args.cpu_pairs)
+        log(f"[bench] cpu baseline: {out['cpu_baseline']}")
+    del batch
     if not args.no_c1:
         out["c1"] = c1_leg(args.steps, not args.no_cpu)
         log(f"[bench] C1: {out['c1']}")
     if not args.no_c2:
-        keep = {}
-        out["c2"] = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads,
-                           keep)
+        out["c2"] = c2_leg(args.c2_reads, args.steps, 0 if args.no_cpu else args.c2_cpu_reads)
         log(f"[bench] C2: {out['c2']}")
-        if not args.no_c5:
-            out["c5"] = c5_leg(keep["index"], keep["seqs"], args.c5_requests)
-            out["c5"]["index"] = "C2 index (50 Mbp / 24 sequences, k=12) resident"
+    if not args.no_c4:
+        out["c4"] = c4_leg(args.c4_reads, args.c4_batch, 0 if args.no_cpu else args.c4_parity,
+                           0 if args.no_c5 else args.c5_requests, not args.no_graph)
+        c5 = out["c4"].pop("c5", None)
+        if c5 is not None:
+            out["c5"] = c5
+        log(f"[bench] C4: {out['c4']}")
+        if c5 is not None:
             log(f"[bench] C5: {out['c5']}")
-        del keep
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(qbuf, qoff, qlen, rbuf, roff, rlen, args.cpu_pairs)
-        log(f"[bench] cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
 

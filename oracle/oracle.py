@@ -291,6 +291,36 @@ def pipeline(index: Index, reads, max_hits=3, match=2, mismatch=-1, gap_open=-10
     return out
 
 
+def pipeline_mt(index: Index, reads, nthreads=16, max_hits=3, match=2, mismatch=-1,
+                gap_open=-10, gap_ext=-1):
+    """pipeline() over `nthreads` host threads, each taking a contiguous
+    share of the reads (tie rule "first", so records depend only on their
+    own read): the same records in the same order as pipeline(), as the
+    reference's CPU path runs its phases on 16 and 8 threads
+    (src/search.cu:323-345, src/align.cu:2455)."""
+    import threading
+    n = len(reads)
+    nthreads = max(1, min(nthreads, n))
+    bounds = [n * t // nthreads for t in range(nthreads + 1)]
+    parts = [None] * nthreads
+
+    def work(t):
+        parts[t] = pipeline(index, reads[bounds[t]:bounds[t + 1]], max_hits, match, mismatch,
+                            gap_open, gap_ext, TIE_FIRST)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in range(nthreads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    out = []
+    for t in range(nthreads):
+        for d in parts[t]:
+            d["read"] += bounds[t]
+            out.append(d)
+    return out
+
+
 # ---------------------------------------------------------------- output text
 def format_record(qry_name: str, ref_name: str, score: float, ref_distance: int,
                   start: int, end: int, qry_cons: bytes, ref_cons: bytes) -> str:

@@ -271,3 +271,42 @@ def sample_reads_fast(rng, genome, off, lens, n, length, sub):
         out[rc] = lut[out[rc][:, ::-1]]
     return (out.reshape(-1), np.arange(n, dtype=np.int64) * length,
             np.full(n, length, np.int32))
+
+
+def sample_reads_var(rng, genome, off, lens, n, lmin, lmax, sub):
+    """n reads of length U{lmin..lmax} at uniform positions inside one
+    sequence each, uniform strand, substitutions at rate `sub` (vectorised;
+    no indels).  Returns (packed uint8 buffer, int64 offsets, int32
+    lengths)."""
+    rl = rng.integers(lmin, lmax + 1, size=n).astype(np.int32)
+    w = (lens - lmax).astype(np.float64)
+    which = rng.choice(len(lens), size=n, p=w / w.sum())
+    pos = off[which] + (rng.random(n) * (lens[which] - lmax)).astype(np.int64)
+    out = genome[pos[:, None] + np.arange(lmax)[None, :]]
+    m = rng.random(out.shape) < sub
+    if m.any():
+        code = np.searchsorted(BASES, out[m])
+        out[m] = BASES[(code + rng.integers(1, 4, m.sum())) % 4]
+    # reverse strand: the reverse complement of the read's own bases
+    rc = rng.random(n) < 0.5
+    if rc.any():
+        lut = np.full(256, ord("A"), np.uint8)
+        lut[ord("A")], lut[ord("C")], lut[ord("G")] = ord("T"), ord("G"), ord("C")
+        idx = np.nonzero(rc)[0]
+        cols = rl[idx][:, None] - 1 - np.arange(lmax)[None, :]
+        flipped = np.take_along_axis(out[idx], np.maximum(cols, 0), axis=1)
+        out[idx] = np.where(cols >= 0, lut[flipped], out[idx])
+    keep = np.arange(lmax)[None, :] < rl[:, None]
+    roff = np.zeros(n, np.int64)
+    roff[1:] = np.cumsum(rl.astype(np.int64))[:-1]
+    return out[keep], roff, rl
+
+
+def c5_requests(rng, genome, off, lens, n_requests, max_reads=4096, lmin=50, lmax=300,
+                err=0.015):
+    """Serving requests of config 5: log-uniform 1..max_reads reads each, read
+    lengths U{lmin..lmax}, substitutions at rate `err`, every read drawn
+    afresh.  Returns a list of (buffer, offsets, lengths) per request."""
+    sizes = np.exp(rng.uniform(0, np.log(max_reads), n_requests)).astype(int)
+    sizes = np.clip(sizes, 1, max_reads)
+    return [sample_reads_var(rng, genome, off, lens, int(k), lmin, lmax, err) for k in sizes]

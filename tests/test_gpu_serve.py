@@ -1,9 +1,12 @@
 """Serving loop on the GPU: requests batched together (continuous batching
-across concurrent slots) get exactly the records they get alone."""
+across concurrent slots) get exactly the records they get alone, and those
+are the oracle's CPU pipeline records over the request."""
 import threading
 
 import numpy as np
 import pytest
+
+from parity_util import assert_records_match
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +25,7 @@ def world():
             reads.append(synth.random_bases(rng, L).tobytes())
         reqs.append(reads)
     ix = search.Index.from_sequences([s.tobytes() for s in seqs], 12, 1000)
-    return ix, reqs
+    return ix, reqs, [s.tobytes() for s in seqs]
 
 
 def same(a, b, ctx):
@@ -32,9 +35,9 @@ def same(a, b, ctx):
     assert a.qry_cons == b.qry_cons and a.ref_cons == b.ref_cons, ctx
 
 
-def test_results_same_alone_and_in_mix(world):
+def test_results_same_alone_and_in_mix(world, orc):
     from swift_amd.serve import Server
-    ix, reqs = world
+    ix, reqs, _ = world
     alone = [ix.align_reads(r, 3) for r in reqs]
     with Server(ix, nslots=4, mem_budget=16 << 30, max_batch_reads=4096) as srv:
         ids = [None] * len(reqs)
@@ -54,12 +57,15 @@ def test_results_same_alone_and_in_mix(world):
     assert st["batches"] < len(reqs) and st["max_coalesced"] > 1  # requests were coalesced
     for i, (g, a) in enumerate(zip(got, alone)):
         same(g, a, i)
+    oix = orc.Index(world[2], 12, 1000)
+    for i, (g, r) in enumerate(zip(got, reqs)):
+        assert_records_match(g, orc.pipeline_mt(oix, r, 8), i)
 
 
 def test_small_budget_many_batches(world):
     from swift_amd.serve import Server
     from swift_amd._lib import SwiftError
-    ix, reqs = world
+    ix, reqs, _ = world
     small = [r for r in reqs if len(r) <= 64][:10]
     with Server(ix, nslots=2, mem_budget=1 << 30, max_batch_reads=128) as srv:
         ids = [srv.submit(r) for r in small]
@@ -72,7 +78,7 @@ def test_small_budget_many_batches(world):
 
 def test_one_slot_in_order(world):
     from swift_amd.serve import Server
-    ix, reqs = world
+    ix, reqs, _ = world
     with Server(ix, nslots=1, mem_budget=8 << 30, max_batch_reads=4096) as srv:
         for r in reqs[:6]:
             got, lat = srv.result(srv.submit(r))
