@@ -925,6 +925,396 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
     }
 }
 
+// ---------------------------------------------------------------------------
+// Filter tier (reads up to MAXQ bases, 32-bit cluster ids): one THREADS-wide
+// workgroup per strand of up to CAP hits, in one pass.  Most hits of a
+// strand are clusters of one (random matches of its k-mers across the
+// genome); only a cluster id seen at least twice can be in a cluster of two
+// or more.  So every hit first sets the bit hash(cid) of a bitmap A in LDS
+// (atomic or); a hit that finds its bit already set sets the same bit of a
+// bitmap B.  After a barrier a hit is a candidate iff its bit in B is set:
+// that holds for every hit of every cluster of size >= 2 (the bit depends on
+// the cluster id alone) and for a few clusters of one whose bit collides.
+// Only candidates are counted exactly, in a small LDS table (count by atomic
+// add, smallest query offset by atomic min).  If the largest count M is at
+// least 2, the biggest clusters are the table's clusters of count M, each
+// listed once through the hit whose insertion returned M; otherwise every
+// cluster is a single hit and the first max_hits clusters are the hits with
+// the smallest cluster ids.  Strands with more candidate hits than the
+// table holds go on to the hash tier (`ovl`), strands past CAP hits to the
+// sort tier (`mid`).
+//
+// Hits are flattened over the workgroup (hit e by thread e % THREADS); the
+// k-mer slot of hit e comes from an owner map (slot + 1 written at the
+// slot's first hit, then a workgroup max-scan) kept in bitmap A's words
+// before the bitmaps are needed.
+template <int THREADS, int CAP, int SMALL, int SMALLCAP, int MAXQ, int WPE>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
+map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
+                   int32_t *mid_count) {
+    constexpr int NW = THREADS / 64;
+    constexpr int PER = (MAXQ + THREADS) / THREADS;
+    constexpr int NS = THREADS * PER;            // k-mer slots, s = thread * PER + u
+    constexpr int HPT = CAP / THREADS;           // hits per thread
+    constexpr int kBits = 1 << 16;               // bits of each bitmap
+    constexpr int kWords = kBits / 32;
+    constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+    constexpr int kCand = 64;
+    static_assert(CAP % THREADS == 0 && HPT % 2 == 0 && CAP / 2 <= 2 * kWords,
+                  "the owner map (16 bits per hit) fits bitmap A and B's words");
+    static_assert(NS < 65535 && MAXQ < 1024 && HPT % 3 == 0,
+                  "owner entries; three 10-bit query offsets per register");
+    static_assert(4 * SMALLCAP <= 3 * SMALL && (SMALL & (SMALL - 1)) == 0,
+                  "candidate clusters fill <= 3/4 of a power-of-two table");
+    __shared__ __align__(16) uint32_t bm[2 * kWords];   // bitmaps A | B; first the owner map
+    __shared__ uint32_t skey[SMALL], scnt[SMALL], smin[SMALL];
+    __shared__ uint32_t ccid[SMALLCAP];          // candidate hits: cluster id,
+    __shared__ uint32_t cqc[SMALLCAP];           // offset | count returned by its insertion << 16
+    __shared__ unsigned char qb[MAXQ + 1];
+    __shared__ int32_t sbase[NS];                // first entry index of slot s minus its offset
+    __shared__ int32_t wsum[NW];
+    __shared__ uint32_t wmax[NW];
+    __shared__ uint64_t red[NW];
+    __shared__ uint64_t cand[kCand];
+    __shared__ uint32_t sh_best;
+    __shared__ int32_t sh_ncand, sh_nhit;
+    __shared__ uint32_t scid0[kMaxIndexSeqs];
+    const int n = 2 * a.rd.nreads;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i = t; i < 2 * kWords; i += THREADS) bm[i] = 0;
+    for (int i = t; i < SMALL; i += THREADS) {
+        skey[i] = kEmpty;
+        scnt[i] = 0;
+        smin[i] = 0xFFFFFFFFu;
+    }
+    for (int i = t; i < a.ix.nseq; i += THREADS) scid0[i] = a.ix.cid0[i];
+    uint16_t *owner = reinterpret_cast<uint16_t *>(bm);
+    // the 4-byte entries through a buffer resource: one 32-bit offset register
+    // per load in flight (the index holds < 2^32 bytes of them when cluster
+    // ids fit 32 bits; loads past the range return 0 instead of faulting)
+    const __amdgpu_buffer_rsrc_t gpos_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.ix.gpos, 0, (int)0xFFFFFFFFu, 0x00020000);
+    const int G = gridDim.x;
+    auto fetch = [&](int len, int64_t off, unsigned char (&nb)[PER]) -> int {
+        if (len < 0 || len > a.max_len) {
+            if (t == 0) atomicOr(a.error, 2);
+            len = 0;
+        }
+        const unsigned char *src = (const unsigned char *)a.rd.seq + off;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = t + u * THREADS;
+            nb[u] = i < len ? src[i] : 0;
+        }
+        return len;
+    };
+    auto store = [&](int sidx, int len, const unsigned char (&nb)[PER]) {
+        const int strand = sidx & 1;
+        char *dst = a.strand_seq ? a.strand_seq + (int64_t)sidx * a.qstride : nullptr;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = t + u * THREADS;
+            if (i < len) {
+                const int at = strand ? len - 1 - i : i;
+                const unsigned char b = strand ? rc_byte(nb[u]) : nb[u];
+                qb[at] = b;
+                if (dst) dst[at] = (char)b;
+            }
+        }
+    };
+    auto heads = [&](int len, int (&c)[PER], int (&st)[PER]) {
+        const int nq = len - a.ix.k + 1;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int o = t + u * THREADS;
+            c[u] = 0;
+            st[u] = 0;
+            if (o < nq) {
+                const int2 b = a.ix.bucket[kmer_hash(qb + o, a.ix.k)];
+                st[u] = b.x;
+                c[u] = b.y;
+            }
+        }
+    };
+    auto bit_of = [](uint32_t cid) { return (cid * 2654435761u) >> 16; };  // 16-bit index
+    auto small_slot = [](uint32_t cid) {
+        return (cid * 2246822519u) >> (32 - __builtin_ctz(SMALL));
+    };
+    int c[PER], st[PER], L0 = 0;
+    int nx_len = 0;
+    int64_t nx_off = 0;
+    if ((int)blockIdx.x < n) {
+        const int s0 = blockIdx.x;
+        unsigned char nb[PER];
+        L0 = fetch(a.rd.len[s0 >> 1], a.rd.off[s0 >> 1], nb);
+        store(s0, L0, nb);
+        if ((int)blockIdx.x + G < n) {
+            const int r = (blockIdx.x + G) >> 1;
+            nx_len = a.rd.len[r];
+            nx_off = a.rd.off[r];
+        }
+    }
+    __syncthreads();
+    heads(L0, c, st);
+    for (int sidx = blockIdx.x; sidx < n; sidx += G) {
+        const bool more = sidx + G < n;
+        const int nsidx = more ? sidx + G : 0;
+        unsigned char nb[PER];
+        int nL = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) nb[u] = 0;
+        if (more) {
+            nL = fetch(nx_len, nx_off, nb);
+            if (sidx + 2 * G < n) {
+                const int r = (sidx + 2 * G) >> 1;
+                nx_len = a.rd.len[r];
+                nx_off = a.rd.off[r];
+            }
+        }
+        if (t == 0) {
+            sh_best = 0;
+            sh_ncand = 0;
+            sh_nhit = 0;
+        }
+        int v = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) v += c[u];
+        const int incl = wave_incl_scan(v);
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        int total = 0, before = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            before += w < wv ? wsum[w] : 0;
+            total += wsum[w];
+        }
+        if (t == 0) a.out.nhits[sidx] = total;
+        if (total > CAP) {
+            if (t == 0) mid[atomicAdd(mid_count, 1)] = sidx;
+            store(nsidx, nL, nb);
+            __syncthreads();
+            heads(nL, c, st);
+            continue;
+        }
+        // owner map: slot + 1 at each non-empty slot's first hit, then a
+        // max-scan (thread t: hits [t * HPT, t * HPT + HPT))
+        {
+            int run = before + incl - v;
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                if (c[u] > 0) owner[run] = (uint16_t)(t * PER + u + 1);
+                sbase[t * PER + u] = st[u] - run;
+                run += c[u];
+            }
+        }
+        __syncthreads();
+        uint32_t ow[HPT / 2];
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < HPT / 2; ++j) {
+            ow[j] = bm[t * (HPT / 2) + j];
+            m = max(m, max(ow[j] & 0xFFFFu, ow[j] >> 16));
+        }
+        uint32_t wm = m;  // wave inclusive max-scan of the thread maxima
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)wm, d, 64);
+            if (lane >= d) wm = max(wm, o);
+        }
+        if (lane == 63) wmax[wv] = wm;
+        __syncthreads();
+        uint32_t carry = (uint32_t)__shfl_up((int)wm, 1, 64);
+        if (lane == 0) carry = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+            if (w < wv) carry = max(carry, wmax[w]);
+        if (t * HPT < total) {
+#pragma unroll
+            for (int j = 0; j < HPT / 2; ++j) {
+                const uint32_t lo = max(carry, ow[j] & 0xFFFFu);
+                const uint32_t hi = max(lo, ow[j] >> 16);
+                carry = hi;
+                bm[t * (HPT / 2) + j] = lo | (hi << 16);
+            }
+        }
+        __syncthreads();
+        // the hits: hit k of this thread (e = t + k * THREADS) exists iff
+        // e < total; its query offset (10 bits, three to a register) and
+        // 4-byte global position, which less the offset is the cluster id
+        uint32_t hcid[HPT], hqp[HPT / 3];
+        auto has = [&](int k) { return t + k * THREADS < total; };
+        auto qof = [&](int k) { return (int)((hqp[k / 3] >> (10 * (k % 3))) & 0x3FFu); };
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            const int e = t + k * THREADS;
+            hcid[k] = 0;
+            if (k % 3 == 0) hqp[k / 3] = 0;
+            if (e < total) {
+                const int sl = (int)owner[e] - 1;
+                hcid[k] = __builtin_amdgcn_raw_buffer_load_b32(
+                    gpos_rsrc, (uint32_t)(sbase[sl] + e) * 4u, 0, 0);
+                hqp[k / 3] |= (uint32_t)(sl / PER + (sl % PER) * THREADS) << (10 * (k % 3));
+            }
+        }
+        // the next strand's bases (qb is free: this strand's heads are in
+        // registers) and, once every owner entry is read, the bitmaps' zeros
+        store(nsidx, nL, nb);
+        __syncthreads();
+        heads(nL, c, st);  // in flight through the rest of this strand
+        if (t * HPT < total) {
+#pragma unroll
+            for (int j = 0; j < HPT / 2; ++j) bm[t * (HPT / 2) + j] = 0;
+        }
+        __syncthreads();
+        // bitmap A: seen; bitmap B: seen again
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            if (!has(k)) continue;
+            hcid[k] = hcid[k] - (uint32_t)qof(k) + (uint32_t)kShiftBias;
+            const uint32_t b = bit_of(hcid[k]);
+            const uint32_t msk = 1u << (b & 31);
+            if (atomicOr(&bm[b >> 5], msk) & msk) atomicOr(&bm[kWords + (b >> 5)], msk);
+        }
+        __syncthreads();
+        // the candidate hits into a list (order does not matter)
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            if (!has(k)) continue;
+            const uint32_t b = bit_of(hcid[k]);
+            if (bm[kWords + (b >> 5)] & (1u << (b & 31))) {
+                const int at = atomicAdd(&sh_nhit, 1);
+                if (at < SMALLCAP) {
+                    ccid[at] = hcid[k];
+                    cqc[at] = (uint32_t)qof(k);
+                }
+            }
+        }
+        __syncthreads();
+        const int ncand_all = sh_nhit;
+        if (ncand_all > SMALLCAP) {
+            // too many candidates for the table: the hash tier counts them all
+            if (t == 0) ovl[atomicAdd(ovl_count, 1)] = sidx;
+        } else {
+            uint32_t mybest = 0;
+            for (int i = t; i < ncand_all; i += THREADS) {
+                const uint32_t cid = ccid[i];
+                const uint32_t q = cqc[i];
+                uint32_t h = small_slot(cid);
+                for (;;) {
+                    const uint32_t old = atomicCAS(&skey[h], kEmpty, cid);
+                    if (old == kEmpty || old == cid) break;
+                    h = (h + 1) & (SMALL - 1);
+                }
+                const uint32_t cnt = atomicAdd(&scnt[h], 1u) + 1;
+                atomicMin(&smin[h], q);
+                cqc[i] = q | (cnt << 16);
+                mybest = max(mybest, cnt);
+            }
+            mybest = wave_max_u32(mybest);
+            if (lane == 0 && mybest) atomicMax(&sh_best, mybest);
+            __syncthreads();
+            const uint32_t M = sh_best;
+            const int64_t ob = (int64_t)sidx * a.out.stride;
+            auto emit = [&](int r, uint32_t cid, int q) {
+                int ref = 0;  // last sequence whose cid0 <= cid
+                for (int lo = 0, hi = a.ix.nseq - 1; lo <= hi;) {
+                    const int mid2 = (lo + hi) >> 1;
+                    if (scid0[mid2] <= cid) {
+                        ref = mid2;
+                        lo = mid2 + 1;
+                    } else {
+                        hi = mid2 - 1;
+                    }
+                }
+                const int sh = (int)((int64_t)(cid - scid0[ref]) - kShiftBias);
+                a.out.ref[ob + r] = (int8_t)ref;
+                a.out.shift[ob + r] = sh;
+                a.out.pos[ob + r] = sh + q;
+            };
+            // the slot of a listed cluster id (present in the table)
+            auto find = [&](uint32_t cid) {
+                uint32_t h = small_slot(cid);
+                while (skey[h] != cid) h = (h + 1) & (SMALL - 1);
+                return h;
+            };
+            if (M >= 2) {
+                // each cluster of count M once: through the hit whose
+                // insertion returned M
+                for (int i = t; i < ncand_all; i += THREADS) {
+                    if ((cqc[i] >> 16) != M) continue;
+                    const int at = atomicAdd(&sh_ncand, 1);
+                    if (at < kCand) cand[at] = ccid[i];
+                }
+                __syncthreads();
+                const int nbig = sh_ncand;
+                const int take = min(nbig, a.max_hits);
+                if (nbig <= kCand) {
+                    if (wv == 0) {  // first `take` cluster ids: repeated wave minimum
+                        uint32_t kk = lane < nbig ? (uint32_t)cand[lane] : 0xFFFFFFFFu;
+                        for (int r = 0; r < take; ++r) {
+                            uint32_t mn = kk;
+#pragma unroll
+                            for (int s2 = 32; s2 >= 1; s2 >>= 1)
+                                mn = min(mn, (uint32_t)__shfl_xor((int)mn, s2, 64));
+                            if (lane == 0) emit(r, mn, (int)smin[find(mn)]);
+                            if (kk == mn) kk = 0xFFFFFFFFu;  // cluster ids are distinct
+                        }
+                    }
+                } else {
+                    // more tied clusters than the list holds: scan the table
+                    uint32_t last = 0;
+                    for (int r = 0; r < take; ++r) {
+                        uint64_t mine = ~0ull;
+                        for (int i = t; i < SMALL; i += THREADS)
+                            if (scnt[i] == M && (r == 0 || skey[i] > last) && skey[i] < mine)
+                                mine = skey[i];
+                        const uint64_t kmin = block_min_u64<THREADS>(mine, red);
+                        last = (uint32_t)kmin;
+                        if (t == 0) emit(r, last, (int)smin[find(last)]);
+                    }
+                }
+                if (t == 0) {
+                    a.out.nbest[sidx] = take;
+                    a.out.nbig[sidx] = nbig;
+                }
+            } else {
+                // every cluster is one hit: the hits with the smallest cluster
+                // ids (distinct), each emitted by its thread with its offset
+                const int take = min(total, a.max_hits);
+                uint32_t last = 0;
+                for (int r = 0; r < take; ++r) {
+                    uint32_t mine = 0xFFFFFFFFu;
+#pragma unroll
+                    for (int k = 0; k < HPT; ++k)
+                        if (has(k) && (r == 0 || hcid[k] > last)) mine = min(mine, hcid[k]);
+                    last = (uint32_t)block_min_u64<THREADS>(mine, red);
+                    int myq = -1;
+#pragma unroll
+                    for (int k = 0; k < HPT; ++k)
+                        if (has(k) && hcid[k] == last) myq = qof(k);
+                    if (myq >= 0) emit(r, last, myq);
+                }
+                if (t == 0) {
+                    a.out.nbest[sidx] = take;
+                    a.out.nbig[sidx] = total;
+                }
+            }
+        }
+        __syncthreads();
+        // clear the bitmaps and, when it was used, the table
+#pragma unroll
+        for (int i = t; i < 2 * kWords; i += THREADS) bm[i] = 0;
+        if (ncand_all > 0 && ncand_all <= SMALLCAP) {
+            for (int i = t; i < SMALL; i += THREADS) {
+                skey[i] = kEmpty;
+                scnt[i] = 0;
+                smin[i] = 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // strands past kMidCap hits: one 256-thread workgroup each, keys in a
 // power-of-two global scratch slot
 __global__ void __launch_bounds__(256) map_strands_big(MapArgs a) {
@@ -1180,9 +1570,9 @@ hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
             // workgroups per CU), strands past its cap on to the full one
             // through the wave tier's (here unused) overflow list
             constexpr int kWpe = 8;  // waves per SIMD: kWpe / 2 workgroups of 8 waves per CU
-            hipLaunchKernelGGL((map_strands_hash<512, kHashSlots / 2, kHashCap / 2, kShortRead, kWpe>),
-                               dim3(kWpe / 2 * mid_blocks), dim3(512), 0, st, a, nullptr, nullptr,
-                               a.ovl_list, a.ovl_count);
+            hipLaunchKernelGGL((map_strands_filter<512, kHashCap, 1024, 768, kShortRead, kWpe>),
+                               dim3(kWpe / 2 * mid_blocks), dim3(512), 0, st, a, a.ovl_list,
+                               a.ovl_count, a.mid_list, a.mid_count);
             hipLaunchKernelGGL((map_strands_hash<512, kHashSlots, kHashCap, kShortRead, 4>),
                                dim3(2 * mid_blocks), dim3(512), 0, st, a, a.ovl_list, a.ovl_count,
                                a.mid_list, a.mid_count);

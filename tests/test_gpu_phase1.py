@@ -133,8 +133,9 @@ def test_map_reads_overflow_strands(S, orc, mode):
     """Strands past the wave's LDS sort (256 hits) take the hash-cluster
     tier up to 6144 hits, the workgroup LDS sort past that, and the global
     scratch path past 16384 hits; every tier in one call.  Mode 1 starts
-    every strand in the hash tier: for reads this short, its half-size table
-    first (up to 3072 hits), the full one for the strands past that."""
+    every strand in the hash tier: for reads this short, its filter stage
+    first (up to 6144 hits, exact counts for repeated cluster ids only), the
+    full table for strands with more repeated ids than that stage holds."""
     rnd = random.Random(4)
     unit = b"ACGTTGCAAC"
     seqs = [unit * 2000, bytes(rnd.choice(b"ACGT") for _ in range(20000)) + unit * 300]
@@ -155,10 +156,11 @@ def test_map_reads_overflow_strands(S, orc, mode):
 
 
 def test_map_hash_table_sizes(S, orc):
-    """Hash-first mapping of short reads passes strands on by hit count: up
-    to 3072 the half-size table, up to 6144 the full one, then the workgroup
-    sort.  Strands of 15, 1600, 3200, 4800, 5000 and 6600 hits (counts from
-    the oracle) land in every stage, on both sides of each hand-over."""
+    """Hash-first mapping of short reads passes strands on: up to 6144 hits
+    the filter stage (strands with more repeated cluster ids than its table
+    holds go on to the full hash table), past 6144 the workgroup sort.
+    Strands of 15, 1600, 3200, 4800, 5000 and 6600 hits (counts from the
+    oracle) cover each hand-over."""
     rnd = random.Random(11)
     ua, ub = b"ACGTTGCAAC", b"TTGACCGTAG"
     seqs = [ua * 5000, ub * 1600, bytes(rnd.choice(b"ACGT") for _ in range(20000))]
@@ -174,6 +176,50 @@ def test_map_hash_table_sizes(S, orc):
         S.set_map_mode(before)
     assert sorted(got.nhits[0::2]) == [15, 1600, 3200, 4800, 5000, 6600]
     check_strands(orc.Index(seqs, 10, 100000), reads, got, 3, orc)
+
+
+def filter_tier_reads():
+    """Workloads of the hash-first filter tier's paths (all counted by the
+    oracle on the CPU): (a) k=6 over 300 kbp, about 1,700 hits per strand,
+    a true locus of about 25 hits among singleton clusters; (b) random reads
+    on a k=8 index of 1 Mbp, a few hundred hits that are all clusters of one;
+    (c) a 24-base motif at 100 positions on the k=12 tuple grid: 100
+    clusters tied at two hits, more than one wave lists."""
+    rnd = random.Random(123)
+    out = []
+    g6 = bytes(rnd.choice(b"ACGT") for _ in range(300_000))
+    reads = [g6[p:p + 150] for p in (rnd.randrange(0, 299_000) for _ in range(40))]
+    reads += [bytes(rnd.choice(b"ACGT") for _ in range(150)) for _ in range(10)]
+    out.append(([g6], 6, reads))
+    g8 = bytes(rnd.choice(b"ACGT") for _ in range(1_000_000))
+    out.append(([g8], 8, [bytes(rnd.choice(b"ACGT") for _ in range(120)) for _ in range(40)]))
+    motif = bytes(rnd.choice(b"ACGT") for _ in range(24))
+    gm = b"".join(motif + bytes(rnd.choice(b"ACGT") for _ in range(120)) for _ in range(100))
+    reads = [bytes(rnd.choice(b"ACGT") for _ in range(30)) + motif +
+             bytes(rnd.choice(b"ACGT") for _ in range(40)) for _ in range(6)]
+    out.append(([gm], 12, reads))
+    return out
+
+
+def test_map_filter_tier_paths(S, orc):
+    """Hash-first mapping (mode 1) of short reads in the filter tier: the
+    biggest clusters of two or more hits from the exact candidate table
+    (listed by one wave, or by table scans past 64 tied clusters), and
+    strands whose clusters are all single hits; every strand against the
+    oracle, tie-set sizes and hit counts included."""
+    before = S.get_map_mode()
+    try:
+        S.set_map_mode(1)
+        seen = set()
+        for seqs, k, reads in filter_tier_reads():
+            ix = S.Index.from_sequences(seqs, k, 100000)
+            got = ix.map_reads(reads, 3)
+            oix = orc.Index(seqs, k, 100000)
+            check_strands(oix, reads, got, 3, orc)
+            seen.update(int(b) for b in got.nbiggest)
+    finally:
+        S.set_map_mode(before)
+    assert 100 in seen   # the 100 tied clusters of case (c)
 
 
 @pytest.mark.parametrize("mode,long_read", [(0, False), (1, False), (1, True)])
