@@ -767,12 +767,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // of the call (both passes) has finished.  sw_walk_kernel: one lane per pair
 // follows the trace and stores the path as 2-bit moves (16 per word, walk
 // order) plus start/end/length; it touches no sequence bytes, so its
-// dependent loads are the trace granules alone.  sw_expand_kernel: one wave
-// per pair turns the moves into the consensus strings: lane l takes moves
-// [16l, 16l+16), a wave prefix sum of the row/column steps gives each lane
-// its starting cell, and the lanes read and write their stretch of the
-// sequences side by side.  A pair still flagged for the int32 re-run (none
-// after the re-run pass) is skipped.
+// dependent loads are the trace granules alone.  sw_expand_kernel turns the
+// moves into the consensus strings, four pairs per wave (below).  A pair
+// still flagged for the int32 re-run (none after the re-run pass) is
+// skipped.
 __global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t *order,
                                                       const int32_t *first_p,
                                                       const int32_t *count, uint32_t *moves,
@@ -803,134 +801,112 @@ __global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t
     }
 }
 
-// What sw_expand_kernel reads of a pair before expanding it.
-struct ExpandMeta {
-    int p, flags, row, col, len, ql, rl;
-    int64_t co, qo, ro;
-};
-__device__ __forceinline__ ExpandMeta expand_meta(const BatchArgs &a, int p) {
-    const PairResult r = a.res[p];
-    return ExpandMeta{p, r.flags, r.row, r.col, a.alen_out[p], a.qlen[p], a.rlen[p], a.coff[p],
-                      a.qoff[p], a.roff[p]};
-}
-
+// sw_expand_kernel: four pairs per wave, one per 16-lane row of lanes.  A
+// pair's moves are taken in chunks of 256 (16 per lane); a prefix sum of
+// the row and column steps over the row's 16 lanes gives each lane the
+// cell its 16 moves start from, and the lane builds its 16 bytes of each
+// string in registers from the pair's bases staged in LDS, then stores
+// them as dwords (bytes at the string's end, then the NUL).
 __global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32_t *order,
                                                         const int32_t *first_p,
                                                         const int32_t *count,
                                                         const uint32_t *moves, int move_words) {
-    // per wave: the pair's query and reference bytes (from the aligned dword
-    // holding the first one), then both output strings
-    constexpr int kOut = kMaxBatchQ + kMaxBatchR + 4;
     constexpr int kQW = (kMaxBatchQ + 6) / 4, kRW = (kMaxBatchR + 6) / 4;  // dwords staged
-    __shared__ uint32_t lq4[4][kQW], lr4[4][kRW];
-    __shared__ unsigned char oq[4][kOut], orr[4][kOut];
+    constexpr int kPairs = 16;                                           // per workgroup
+    __shared__ uint32_t lq4[kPairs][kQW], lr4[kPairs][kRW];
     const int n = *count;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int slot = threadIdx.x >> 4, sl = threadIdx.x & 15;
     const int32_t *ord = order + (first_p ? *first_p : 0);
-    const int S = gridDim.x * 4;
-    // A pair's order entry is loaded one pair ahead and its metadata during
-    // the previous pair's expansion; its moves and bases are loaded together.
-    int i = blockIdx.x * 4 + wv;
-    ExpandMeta m{};
-    if (i < n) m = expand_meta(a, ord[i]);
-    for (; i < n; i += S) {
-        const int pn = i + S < n ? ord[i + S] : -1;
-        if (m.flags & 1) {  // left to the int32 pass
-            if (pn >= 0) m = expand_meta(a, pn);
-            continue;
+    for (int i0 = blockIdx.x * kPairs; i0 < n; i0 += gridDim.x * kPairs) {
+        const int i = i0 + slot;
+        int p = -1, len = 0, row = 0, col = 0, nq = 0, nr = 0;
+        int64_t co = 0;
+        const unsigned char *qsrc = nullptr, *rsrc = nullptr;
+        if (i < n) {
+            p = ord[i];
+            const PairResult r = a.res[p];
+            if (r.flags & 1) {
+                p = -1;  // left to the int32 pass
+            } else {
+                len = a.alen_out[p];
+                row = r.row;
+                col = r.col;
+                // the path only visits rows <= row and columns <= col
+                nq = min(a.qlen[p], row);
+                nr = min(a.rlen[p], col);
+                co = a.coff[p];
+                qsrc = (const unsigned char *)a.qseq + a.qoff[p];
+                rsrc = (const unsigned char *)a.rseq + a.roff[p];
+            }
         }
-        const int len = m.len;
-        char *qc = a.qcons + m.co, *rc = a.rcons + m.co;
-        // the path only visits rows <= row and columns <= col
-        const int nq = min(m.ql, m.row), nr = min(m.rl, m.col);
-        const int nw = (len + 15) >> 4;
-        const uint32_t w = lane < nw ? moves[(int64_t)m.p * move_words + lane] : 0u;
-        const uintptr_t qa = (uintptr_t)(a.qseq + m.qo), ra = (uintptr_t)(a.rseq + m.ro);
-        const int qm = (int)(qa & 3), rm = (int)(ra & 3);
-        const uint32_t *qs4 = (const uint32_t *)(qa - qm), *rs4 = (const uint32_t *)(ra - rm);
-        // an aligned dword holding a sequence byte lies within that byte's page
-        // (no dword is read for an empty stretch)
+        // the bases into LDS, from the aligned dwords holding them (an aligned
+        // dword holding a sequence byte lies within that byte's page)
+        const int qm = (int)((uintptr_t)qsrc & 3), rm = (int)((uintptr_t)rsrc & 3);
         const int nqw = nq > 0 ? (qm + nq + 3) >> 2 : 0, nrw = nr > 0 ? (rm + nr + 3) >> 2 : 0;
-        uint32_t vq[(kQW + 63) / 64], vr[(kRW + 63) / 64];
-#pragma unroll
-        for (int u = 0; u < (kQW + 63) / 64; ++u)
-            vq[u] = lane + 64 * u < nqw ? qs4[lane + 64 * u] : 0u;
-#pragma unroll
-        for (int u = 0; u < (kRW + 63) / 64; ++u)
-            vr[u] = lane + 64 * u < nrw ? rs4[lane + 64 * u] : 0u;
-        ExpandMeta mn = m;
-        if (pn >= 0) mn = expand_meta(a, pn);
-#pragma unroll
-        for (int u = 0; u < (kQW + 63) / 64; ++u)
-            if (lane + 64 * u < nqw) lq4[wv][lane + 64 * u] = vq[u];
-#pragma unroll
-        for (int u = 0; u < (kRW + 63) / 64; ++u)
-            if (lane + 64 * u < nrw) lr4[wv][lane + 64 * u] = vr[u];
-        const unsigned char *lq = (const unsigned char *)lq4[wv] + qm;
-        const unsigned char *lr = (const unsigned char *)lr4[wv] + rm;
-        const int cnt = lane < nw ? min(16, len - 16 * lane) : 0;
-        // row steps: diagonal and up moves; column steps: diagonal and left
-        int di = 0, dj = 0;
-        for (int k = 0; k < cnt; ++k) {
-            const int op = (w >> (2 * k)) & 3;
-            di += op != 2;
-            dj += op != 1;
-        }
-        int ei = di, ej = dj;  // inclusive wave scans
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            int oi = __shfl_up(ei, d, 64), oj = __shfl_up(ej, d, 64);
-            if (lane >= d) {
-                ei += oi;
-                ej += oj;
-            }
-        }
-        wave_lds_sync();  // this wave's lq/lr filled
-        int ci = m.row - (ei - di), cj = m.col - (ej - dj);
-        for (int k = 0; k < cnt; ++k) {
-            const int op = (w >> (2 * k)) & 3;
-            const int at = 16 * lane + k;
-            oq[wv][at] = op == 2 ? '-' : lq[ci - 1];
-            orr[wv][at] = op == 1 ? '-' : lr[cj - 1];
-            ci -= op != 2;
-            cj -= op != 1;
-        }
-        if (lane == 0) {
-            oq[wv][len] = 0;
-            orr[wv][len] = 0;
-        }
+        for (int u = sl; u < nqw; u += 16)
+            lq4[slot][u] = reinterpret_cast<const uint32_t *>(qsrc - qm)[u];
+        for (int u = sl; u < nrw; u += 16)
+            lr4[slot][u] = reinterpret_cast<const uint32_t *>(rsrc - rm)[u];
         wave_lds_sync();
-        // coalesced copy out (NUL included); dword stores when aligned
-        if ((m.co & 3) == 0) {
-            const int nd = (len + 1 + 3) >> 2;
-            for (int k = lane; k < nd; k += 64) {
-                uint32_t vq = 0, vr = 0;
+        const unsigned char *lq = (const unsigned char *)lq4[slot] + qm;
+        const unsigned char *lr = (const unsigned char *)lr4[slot] + rm;
+        char *qc = a.qcons + co, *rc = a.rcons + co;
+        const bool aligned = (co & 3) == 0;
+        int ci = row, cj = col;
+        const int nchunk = (len + 255) >> 8;
+        for (int c = 0; c < nchunk; ++c) {
+            const int k0 = c * 256 + sl * 16;
+            const int cnt = max(0, min(16, len - k0));
+            const uint32_t w = cnt > 0 ? moves[(int64_t)p * move_words + (k0 >> 4)] : 0u;
+            // row steps: every move but left (2); column steps: every move but
+            // up (1); ops are 0, 1 or 2
+            const uint32_t lo2 = cnt == 16 ? 0x55555555u : ((1u << (2 * cnt)) - 1u) & 0x55555555u;
+            const uint32_t hib = (w >> 1) & lo2, lob = w & lo2;
+            const int di = cnt - __builtin_popcount(hib & ~lob);
+            const int dj = cnt - __builtin_popcount(lob & ~hib);
+            int ei = di, ej = dj;  // inclusive scans over the row's 16 lanes
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int at = 4 * k + b;
-                    if (at <= len) {
-                        vq |= (uint32_t)oq[wv][at] << (8 * b);
-                        vr |= (uint32_t)orr[wv][at] << (8 * b);
-                    }
-                }
-                if (4 * k + 3 <= len) {
-                    reinterpret_cast<uint32_t *>(qc)[k] = vq;
-                    reinterpret_cast<uint32_t *>(rc)[k] = vr;
-                } else {
-                    for (int b = 0; 4 * k + b <= len; ++b) {
-                        qc[4 * k + b] = (char)(vq >> (8 * b));
-                        rc[4 * k + b] = (char)(vr >> (8 * b));
-                    }
+            for (int d = 1; d < 16; d <<= 1) {
+                const int oi = __shfl_up(ei, d, 16), oj = __shfl_up(ej, d, 16);
+                if (sl >= d) {
+                    ei += oi;
+                    ej += oj;
                 }
             }
-        } else {
-            for (int k = lane; k <= len; k += 64) {
-                qc[k] = (char)oq[wv][k];
-                rc[k] = (char)orr[wv][k];
+            int xi = ci - (ei - di), xj = cj - (ej - dj);
+            uint32_t oq[4] = {0u, 0u, 0u, 0u}, orr[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (k < cnt) {
+                    const int op = (w >> (2 * k)) & 3;
+                    const uint32_t qb = op == 2 ? (uint32_t)'-' : (uint32_t)lq[xi - 1];
+                    const uint32_t rb = op == 1 ? (uint32_t)'-' : (uint32_t)lr[xj - 1];
+                    oq[k >> 2] |= qb << (8 * (k & 3));
+                    orr[k >> 2] |= rb << (8 * (k & 3));
+                    xi -= op != 2;
+                    xj -= op != 1;
+                }
             }
+            if (cnt == 16 && aligned) {
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    reinterpret_cast<uint32_t *>(qc + k0)[d] = oq[d];
+                    reinterpret_cast<uint32_t *>(rc + k0)[d] = orr[d];
+                }
+            } else {
+                for (int k = 0; k < cnt; ++k) {
+                    qc[k0 + k] = (char)(oq[k >> 2] >> (8 * (k & 3)));
+                    rc[k0 + k] = (char)(orr[k >> 2] >> (8 * (k & 3)));
+                }
+            }
+            ci -= __shfl(ei, 15, 16);
+            cj -= __shfl(ej, 15, 16);
         }
-        wave_lds_sync();  // before the next pair overwrites this wave's buffers
-        m = mn;
+        if (p >= 0 && sl == 0) {
+            qc[len] = 0;
+            rc[len] = 0;
+        }
+        wave_lds_sync();  // before the next pairs overwrite this wave's buffers
     }
 }
 
@@ -1279,8 +1255,8 @@ hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t 
     if (max_pairs <= 0) return hipSuccess;
     hipLaunchKernelGGL(sw_walk_kernel, dim3((max_pairs + 255) / 256), dim3(256), 0, st, ba, order,
                        first, count, moves, move_words);
-    hipLaunchKernelGGL(sw_expand_kernel, dim3((max_pairs + 3) / 4), dim3(256), 0, st, ba, order,
-                       first, count, moves, move_words);
+    hipLaunchKernelGGL(sw_expand_kernel, dim3((max_pairs + 15) / 16), dim3(256), 0, st, ba,
+                       order, first, count, moves, move_words);
     return hipGetLastError();
 }
 
