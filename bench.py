@@ -36,7 +36,7 @@ PEAK_INT32_TOPS = 39.3
 # from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) counts, each scaled by
 # the same counter's reading for a device copy of known size
 # (profiles/sw_batch_c3_pmc.txt)
-TRAFFIC_B_PER_CELL = 0.6161
+TRAFFIC_B_PER_CELL = 0.6308
 
 
 def log(*a):
