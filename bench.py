@@ -373,9 +373,9 @@ def c5_leg(index, g, off, lens, n_requests, seed=20261205, p1_per_kmer_read=None
     """Serving: requests of log-uniform 1..4096 reads (lengths U{50..300},
     1.5 % substitutions, every read drawn afresh; synth.c5_requests)
     submitted to a swift_server with the index resident.  First every
-    request at once (capacity, reads/s), then Poisson arrivals at 70 % of
-    that capacity; latency = submit -> records ready, measured by the
-    server."""
+    request at once (capacity, reads/s), then Poisson arrivals at the fixed
+    rate C5_RATE (the headline p50/p99) and at 70 % of the measured
+    capacity; latency = submit -> records ready, measured by the server."""
     from swift_amd import synth
     rng = np.random.default_rng(seed)
     packed = synth.c5_requests(rng, g, off, lens, n_requests)
@@ -408,6 +408,10 @@ def c5_leg(index, g, off, lens, n_requests, seed=20261205, p1_per_kmer_read=None
     return res
 
 
+# the fixed arrival rate of the serving leg's headline latencies (reads/s)
+C5_RATE = 1_000_000
+
+
 def _c5_serve(index, packed, sizes, total_reads, n_requests, rng, res, seq_lens):
     import threading
     from swift_amd.serve import Server
@@ -421,81 +425,116 @@ def _c5_serve(index, packed, sizes, total_reads, n_requests, rng, res, seq_lens)
         res["capacity_cells"] = sum(record_cells(r, p[2], seq_lens)
                                     for r, p in zip(recs, packed))
         capacity = total_reads / cap_s
-        lam = 0.7 * capacity
         mean_size = total_reads / n_requests
-        rate = lam / mean_size                      # requests per second
-        gaps = rng.exponential(1.0 / rate, n_requests)
-        lat = [0.0] * n_requests
-        ids = [None] * n_requests
-        ready = threading.Semaphore(0)
 
-        def collect():
+        def poisson_run(lam):
+            """Poisson arrivals of the requests at lam reads/s; latency =
+            submit -> records ready, measured by the server."""
+            rate = lam / mean_size                      # requests per second
+            gaps = rng.exponential(1.0 / rate, n_requests)
+            lat = [0.0] * n_requests
+            ids = [None] * n_requests
+            ready = threading.Semaphore(0)
+
+            def collect():
+                for k in range(n_requests):
+                    ready.acquire()
+                    _, lat[k] = srv.result(ids[k], strings=False)
+
+            th = threading.Thread(target=collect)
+            th.start()
+            b0 = srv.stats()["batches"]
+            t0 = time.perf_counter()
+            due = t0
             for k in range(n_requests):
-                ready.acquire()
-                _, lat[k] = srv.result(ids[k], strings=False)
+                due += gaps[k]
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[k] = srv.submit_packed(*packed[k])
+                ready.release()
+            th.join()
+            run_s = time.perf_counter() - t0
+            st = srv.stats()
+            lat = np.array(lat)
+            return {"arrival_rate_reads_per_s": round(lam, 1),
+                    "arrival_rate_requests_per_s": round(rate, 2),
+                    "served_reads_per_s": round(total_reads / run_s, 1),
+                    "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                    "mean_ms": round(float(lat.mean()), 3),
+                    "batches": st["batches"] - b0}
 
-        th = threading.Thread(target=collect)
-        th.start()
-        t0 = time.perf_counter()
-        due = t0
-        for k in range(n_requests):
-            due += gaps[k]
-            d = due - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            ids[k] = srv.submit_packed(*packed[k])
-            ready.release()
-        th.join()
-        run_s = time.perf_counter() - t0
+        fixed = poisson_run(float(C5_RATE))
+        loaded = poisson_run(0.7 * capacity)
         st = srv.stats()
-    lat = np.array(lat)
-    res.update({"capacity_reads_per_s": round(capacity, 1),
-                "capacity_s": round(cap_s, 4),
-                "arrival_rate_reads_per_s": round(lam, 1),
-                "arrival_rate_requests_per_s": round(rate, 2),
-                "served_reads_per_s": round(total_reads / run_s, 1),
-                "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "mean_ms": round(float(lat.mean()), 3),
-                "batches": st["batches"], "max_coalesced": st["max_coalesced"],
-                "slots": 4})
+    res.update({"capacity_reads_per_s": round(capacity, 1), "capacity_s": round(cap_s, 4),
+                **fixed, "at_70pct_capacity": loaded,
+                "max_coalesced": st["max_coalesced"], "slots": 4})
     return res
 
 
-def c4_graph(ix, g, off, lens, rng, n_reads, batch):
-    """The C4 pipeline as one hipGraph replay per batch: `batch` fixed read
-    slots of 150 bases, each batch uploaded into them (upload timed, as for
-    the eager batches), and a 20,000-read batch compared record by record,
-    strings included, with the eager pipeline."""
+def c4_graph(ix, g, off, lens, rng, n_reads, batch, depth=2):
+    """The C4 pipeline as hipGraph replays: `depth` graphs of batch // depth
+    read slots of 150 bases, each on its own stream, take the batches in
+    turn.  A graph's next batch is uploaded (timed, as for the eager
+    batches) once its previous replay has finished, while the other graphs'
+    replays run, so one batch's phase 1 overlaps the phase 2 of the batch
+    before.  The reads are drawn before the timed region.  A 20,000-read
+    batch is compared record by record, strings included, with the eager
+    pipeline."""
     from swift_amd import search, synth
-    from swift_amd._lib import check, lib
-    from swift_amd.device import synchronize
-    pg = search.PipelineGraph(ix, batch, 150, 3)
-    dt, nrec, done = 0.0, 0, 0
+    from swift_amd.device import Stream, synchronize
+    slots = batch // depth
+    pgs = [search.PipelineGraph(ix, slots, 150, 3) for _ in range(depth)]
+    sts = [Stream() for _ in range(depth)]
+    work, done = [], 0
     while done < n_reads:
-        nb = min(batch, n_reads - done)
+        nb = min(slots, n_reads - done)
         buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, nb, 150, 0.01)
-        slot_len = np.zeros(batch, np.int32)
+        slot_len = np.zeros(slots, np.int32)
         slot_len[:nb] = rl
-        synchronize()
-        t0 = time.perf_counter()
-        pg.seq.upload(buf)
-        pg.len.upload(slot_len)
-        pg.launch()
-        synchronize()
-        dt += time.perf_counter() - t0
-        check(lib().swift_pipeline_graph_check(pg._h))
-        nrec += int(pg.out.bufs["nrecords"].download(np.int32, 1)[0])
+        work.append((buf, slot_len))
         done += nb
-        if done % (8 * batch) == 0 or done == n_reads:
-            log(f"[bench] C4 graph {done}/{n_reads} reads, {dt:.2f} s on the device path")
+    nrec, busy = 0, [False] * depth
+
+    def retire(s):
+        nonlocal nrec
+        sts[s].synchronize()
+        pgs[s].check()
+        nrec += int(pgs[s].out.bufs["nrecords"].download(np.int32, 1)[0])
+        busy[s] = False
+
+    synchronize()
+    t0 = time.perf_counter()
+    for b, (buf, slot_len) in enumerate(work):
+        s = b % depth
+        if busy[s]:
+            retire(s)
+        pgs[s].seq.upload(buf)
+        pgs[s].len.upload(slot_len)
+        pgs[s].launch(sts[s].cptr)
+        busy[s] = True
+        if (b + 1) % (16 * depth) == 0:
+            log(f"[bench] C4 graph {(b + 1) * slots}/{n_reads} reads, "
+                f"{time.perf_counter() - t0:.2f} s on the device path")
+    for s in range(depth):
+        if busy[s]:
+            retire(s)
+    dt = time.perf_counter() - t0
+    log(f"[bench] C4 graph {n_reads} reads, {dt:.2f} s on the device path")
+    del work
     buf, ro, rl = synth.sample_reads_fast(rng, g, off, lens, 20000, 150, 0.01)
     reads = [buf[o:o + 150].tobytes() for o in ro]
-    a = pg.run(reads)
+    a = pgs[0].run(reads)
     b = ix.align_reads(reads, 3)
     same = a.n == b.n and all(a.record(i) == b.record(i) for i in range(a.n))
-    pg.close()
+    for pg in pgs:
+        pg.close()
+    for st in sts:
+        st.close()
     return {"reads_per_s": round(n_reads / dt, 1), "records": nrec, "seconds": round(dt, 3),
+            "graphs": depth, "batch_reads": slots,
             "vs_eager": {"reads": len(reads), "records": a.n, "identical": bool(same)}}
 
 
@@ -600,7 +639,7 @@ def main():
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the C4 hipGraph stream")
-    ap.add_argument("--c5-requests", type=int, default=400)
+    ap.add_argument("--c5-requests", type=int, default=1500)
     ap.add_argument("--c4-only", action="store_true", help="only the genome-scale C4 leg "
                     "(and C5 on its index)")
     ap.add_argument("--c4-reads", type=int, default=10_000_000)

@@ -3,10 +3,11 @@
 // memory, for callers that cannot allocate device memory (code under HIP
 // graph capture, tight serving loops, harnesses that forbid hipMalloc).
 //
-// The plain kernel writes each pair's 2-bit traceback into scratch before
-// walking it.  Here that scratch is `scratch`, a file-scope __device__ array
-// of kNoAllocWords 32-bit words (64 MiB) that each GPU holds until the
-// process ends.  Its device address is looked up once per device with
+// The plain path keeps each pair's 2-bit traceback, best cell and walk
+// moves in scratch between its kernels.  Here that scratch is `scratch`, a
+// file-scope __device__ array of kNoAllocWords 32-bit words (64 MiB) that
+// each GPU holds until the process ends.  Its device address is looked up
+// once per device with
 // hipGetSymbolAddress and kept; kernels get that address, never the array's
 // host-side name.  Pairs run in chunks that fit the array, one after another
 // on the caller's stream.  Every call on a GPU shares that GPU's array, so
@@ -81,9 +82,8 @@ int swift_align_sequences_noalloc(char *querySeq_d, char *refSeq_d, int querySeq
     bool f32;
     if ((rc = validate_scoring(sc, qlen, rlen, &f32))) return rc;
     if (npairs == 0) return kOk;
-    // trace words per pair: whole 4-step granules of (rlen + 63) steps
-    const int64_t wpp =
-        (int64_t)((rlen + 63 + 3) & ~3) * plain_trace_words(plain_rows_for(qlen)) * 64;
+    // words per pair: the trace, the best cell and the walk's moves
+    const int64_t wpp = plain_scratch_words(qlen, rlen);
     if (wpp > kNoAllocWords) {
         set_error("one pair's traceback needs %lld words, beyond the %lld-word scratch",
                   (long long)wpp, (long long)kNoAllocWords);
@@ -97,8 +97,7 @@ int swift_align_sequences_noalloc(char *querySeq_d, char *refSeq_d, int querySeq
         const int cn = std::min(chunk, npairs - c0);
         PlainOut out{score_d, alignStart_d, alignEnd_d, alignLength_d, queryConsensus_d,
                      refConsensus_d, (int64_t)qlen + rlen + 2};
-        SWIFT_HIP_TRY(launch_plain(querySeq_d, refSeq_d, qlen, rlen, sc, cn, c0, trace, wpp, out,
-                                   st));
+        SWIFT_HIP_TRY(launch_plain(querySeq_d, refSeq_d, qlen, rlen, sc, cn, c0, trace, out, st));
     }
     return kOk;
 }

@@ -232,7 +232,7 @@ int scoring_from_floats(float match, float mismatch, float gap_open, float gap_e
 // ---------------------------------------------------------------------------
 // Plain path (alignSequences drop-in).
 // ---------------------------------------------------------------------------
-static size_t g_plain_budget_words = (size_t)1 << 28;  // 1 GiB of trace
+static size_t g_plain_budget_words = (size_t)1 << 28;  // 1 GiB of trace, moves and best cells
 
 int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match, float mismatch,
                            float gap_open, float gap_ext, float *score_d, int *start_d,
@@ -259,9 +259,7 @@ int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match
     bool f32;
     if ((rc = validate_scoring(sc, qlen, rlen, &f32))) return rc;
     Workspace &w = ws();
-    const int R = plain_rows_for(qlen);
-    // trace words per pair: whole 4-step granules of (rlen + 63) steps
-    const int64_t wpp = (int64_t)((rlen + 63 + 3) & ~3) * plain_trace_words(R) * 64;
+    const int64_t wpp = plain_scratch_words(qlen, rlen);
     int chunk = (int)std::min<int64_t>(npairs, std::max<int64_t>(1, g_plain_budget_words / wpp));
     if (!w.p_done) SWIFT_HIP_TRY(hipEventCreateWithFlags(&w.p_done, hipEventDisableTiming));
     if (w.p_pending) {
@@ -277,7 +275,7 @@ int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match
         int cn = std::min(chunk, npairs - c0);
         PlainOut out{score_d, start_d, end_d, alen_d, qcons_d, rcons_d, (int64_t)qlen + rlen + 2};
         SWIFT_HIP_TRY(launch_plain(q_d, r_d, qlen, rlen, sc, cn, c0, w.p_trace.as<uint32_t>(),
-                                   wpp, out, st));
+                                   out, st));
     }
     SWIFT_HIP_TRY(hipEventRecord(w.p_done, st));
     w.p_pending = true;

@@ -13,11 +13,15 @@ struct TaskArgs;
 
 // launch helpers implemented in sw_kernels.hip
 struct PlainOut;
+// Plain path over pairs [pair0, pair0 + npairs) of the caller's arrays:
+// sweep, walk and consensus expansion.  scratch holds
+// npairs * plain_scratch_words(qlen, rlen_max) words.
 hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
-                        int npairs, int pair0, uint32_t *trace, int64_t wpp, const PlainOut &out,
+                        int npairs, int pair0, uint32_t *scratch, const PlainOut &out,
                         hipStream_t st);
 int plain_rows_for(int qlen);
 int plain_trace_words(int R);
+int64_t plain_scratch_words(int qlen, int rlen);  // device words per pair
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st);
 
 // up to four device arrays of 32-bit words -> one host staging block

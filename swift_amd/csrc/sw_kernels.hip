@@ -488,8 +488,9 @@ __device__ __forceinline__ void wave_store_fence() {
 }
 
 // ---------------------------------------------------------------------------
-// Plain per-pair kernel: the alignSequences drop-in.  One wave per pair,
-// G = 64, byte semantics, int32.  Inputs use the reference's layout:
+// Plain per-pair kernel: the alignSequences drop-in.  One wave per pair
+// sweeps, G = 64, byte semantics, int32; the traceback follows in
+// sw_plain_walk_kernel and sw_plain_expand_kernel.  Inputs use the reference's layout:
 // pair p's query at querySeq + p*(qlen+1) (all qlen bytes are rows, as in
 // smithWaterman_cpu), reference at refSeq + p*(rlen+1), columns stopping at
 // its first NUL.  LDS holds the reference codes of the pair.
@@ -498,7 +499,7 @@ template <int R>
 __global__ void __launch_bounds__(64)
 sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, int qlen, int rlen_max,
                 KeyConsts kc, int npairs, int pair0, uint32_t *__restrict__ trace,
-                int64_t trace_words_per_pair, PlainOut out) {
+                int64_t trace_words_per_pair, int4 *__restrict__ res) {
     using S = Sweep<64, R, false, kBytes>;
     extern __shared__ uint32_t lds_rr[];
     const int p = blockIdx.x;
@@ -555,18 +556,8 @@ sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, in
         }
     }
     const GroupResult gr = group_best<64>(bh, brow, bcol, R, false, 0, 0);
-    wave_store_fence();
-    if (g == 0) {  // one lane walks the traceback, as the reference's thread 0 does
-        const int gp = pair0 + p;
-        const PairTrace pt{(int64_t)p * trace_words_per_pair, 0, (int16_t)R, (uint8_t)S::W, 0};
-        char *qc = out.qcons + (int64_t)gp * out.cstride;
-        char *rc = out.rcons + (int64_t)gp * out.cstride;
-        WalkOut wo = walk_pair(trace, gr.score, gr.row, gr.col, pt, qs, rs, qlen, rlen, qc, rc);
-        out.score[gp] = (float)gr.score;
-        out.start[gp] = wo.start;
-        out.end[gp] = wo.end;
-        out.alen[gp] = wo.len;
-    }
+    // the walk runs in sw_plain_walk_kernel, one lane per pair
+    if (g == 0) res[p] = make_int4(gr.score, gr.row, gr.col, rlen);
 }
 
 // ---------------------------------------------------------------------------
@@ -801,12 +792,99 @@ __global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t
     }
 }
 
-// sw_expand_kernel: four pairs per wave, one per 16-lane row of lanes.  A
-// pair's moves are taken in chunks of 256 (16 per lane); a prefix sum of
-// the row and column steps over the row's 16 lanes gives each lane the
-// cell its 16 moves start from, and the lane builds its 16 bytes of each
-// string in registers from the pair's bases staged in LDS, then stores
-// them as dwords (bytes at the string's end, then the NUL).
+// One pair's consensus expansion, by one 16-lane row of a wave (lane sl of
+// the row).  A pair's moves are taken in chunks of 256 (16 per lane); a
+// prefix sum of the row and column steps over the row's 16 lanes gives each
+// lane the cell its 16 moves start from, and the lane builds its 16 bytes of
+// each string in registers from the pair's bases staged in LDS, then stores
+// them as dwords where the strings are dword aligned (bytes at the string's
+// end, then the NUL).  live = false: the row only takes part in the wave's
+// LDS ordering.
+struct ExpandPair {
+    bool live = false;
+    int len = 0, row = 0, col = 0;   // moves, best cell (1-based)
+    int nq = 0, nr = 0;              // bases the path can reach
+    const unsigned char *qsrc = nullptr, *rsrc = nullptr;
+    char *qc = nullptr, *rc = nullptr;
+    const uint32_t *mv = nullptr;    // 2-bit moves, 16 per word, walk order
+};
+
+__device__ __forceinline__ void expand_row(const ExpandPair &e, uint32_t *lq4, uint32_t *lr4,
+                                           int sl) {
+    // the bases into LDS, from the aligned dwords holding them (an aligned
+    // dword holding a sequence byte lies within that byte's page)
+    const int qm = (int)((uintptr_t)e.qsrc & 3), rm = (int)((uintptr_t)e.rsrc & 3);
+    const int nqw = e.nq > 0 ? (qm + e.nq + 3) >> 2 : 0;
+    const int nrw = e.nr > 0 ? (rm + e.nr + 3) >> 2 : 0;
+    for (int u = sl; u < nqw; u += 16)
+        lq4[u] = reinterpret_cast<const uint32_t *>(e.qsrc - qm)[u];
+    for (int u = sl; u < nrw; u += 16)
+        lr4[u] = reinterpret_cast<const uint32_t *>(e.rsrc - rm)[u];
+    wave_lds_sync();
+    const unsigned char *lq = (const unsigned char *)lq4 + qm;
+    const unsigned char *lr = (const unsigned char *)lr4 + rm;
+    char *qc = e.qc, *rc = e.rc;
+    const bool aligned = (((uintptr_t)qc | (uintptr_t)rc) & 3) == 0;
+    const int len = e.len;
+    int ci = e.row, cj = e.col;
+    const int nchunk = (len + 255) >> 8;
+    for (int c = 0; c < nchunk; ++c) {
+        const int k0 = c * 256 + sl * 16;
+        const int cnt = max(0, min(16, len - k0));
+        const uint32_t w = cnt > 0 ? e.mv[k0 >> 4] : 0u;
+        // row steps: every move but left (2); column steps: every move but
+        // up (1); ops are 0, 1 or 2
+        const uint32_t lo2 = cnt == 16 ? 0x55555555u : ((1u << (2 * cnt)) - 1u) & 0x55555555u;
+        const uint32_t hib = (w >> 1) & lo2, lob = w & lo2;
+        const int di = cnt - __builtin_popcount(hib & ~lob);
+        const int dj = cnt - __builtin_popcount(lob & ~hib);
+        int ei = di, ej = dj;  // inclusive scans over the row's 16 lanes
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int oi = __shfl_up(ei, d, 16), oj = __shfl_up(ej, d, 16);
+            if (sl >= d) {
+                ei += oi;
+                ej += oj;
+            }
+        }
+        int xi = ci - (ei - di), xj = cj - (ej - dj);
+        uint32_t oq[4] = {0u, 0u, 0u, 0u}, orr[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < cnt) {
+                const int op = (w >> (2 * k)) & 3;
+                const uint32_t qb = op == 2 ? (uint32_t)'-' : (uint32_t)lq[xi - 1];
+                const uint32_t rb = op == 1 ? (uint32_t)'-' : (uint32_t)lr[xj - 1];
+                oq[k >> 2] |= qb << (8 * (k & 3));
+                orr[k >> 2] |= rb << (8 * (k & 3));
+                xi -= op != 2;
+                xj -= op != 1;
+            }
+        }
+        if (cnt == 16 && aligned) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                reinterpret_cast<uint32_t *>(qc + k0)[d] = oq[d];
+                reinterpret_cast<uint32_t *>(rc + k0)[d] = orr[d];
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) {
+                qc[k0 + k] = (char)(oq[k >> 2] >> (8 * (k & 3)));
+                rc[k0 + k] = (char)(orr[k >> 2] >> (8 * (k & 3)));
+            }
+        }
+        ci -= __shfl(ei, 15, 16);
+        cj -= __shfl(ej, 15, 16);
+    }
+    if (e.live && sl == 0) {
+        qc[len] = 0;
+        rc[len] = 0;
+    }
+    wave_lds_sync();  // before the next pairs overwrite this row's buffers
+}
+
+// sw_expand_kernel: the batched pass's consensus strings, four pairs per
+// wave (one per 16-lane row), 16 pairs per workgroup.
 __global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32_t *order,
                                                         const int32_t *first_p,
                                                         const int32_t *count,
@@ -819,95 +897,88 @@ __global__ void __launch_bounds__(256) sw_expand_kernel(BatchArgs a, const int32
     const int32_t *ord = order + (first_p ? *first_p : 0);
     for (int i0 = blockIdx.x * kPairs; i0 < n; i0 += gridDim.x * kPairs) {
         const int i = i0 + slot;
-        int p = -1, len = 0, row = 0, col = 0, nq = 0, nr = 0;
-        int64_t co = 0;
-        const unsigned char *qsrc = nullptr, *rsrc = nullptr;
+        ExpandPair e;
         if (i < n) {
-            p = ord[i];
+            const int p = ord[i];
             const PairResult r = a.res[p];
-            if (r.flags & 1) {
-                p = -1;  // left to the int32 pass
-            } else {
-                len = a.alen_out[p];
-                row = r.row;
-                col = r.col;
+            if (!(r.flags & 1)) {  // a flagged pair is left to the int32 pass
+                e.live = true;
+                e.len = a.alen_out[p];
+                e.row = r.row;
+                e.col = r.col;
                 // the path only visits rows <= row and columns <= col
-                nq = min(a.qlen[p], row);
-                nr = min(a.rlen[p], col);
-                co = a.coff[p];
-                qsrc = (const unsigned char *)a.qseq + a.qoff[p];
-                rsrc = (const unsigned char *)a.rseq + a.roff[p];
+                e.nq = min(a.qlen[p], r.row);
+                e.nr = min(a.rlen[p], r.col);
+                const int64_t co = a.coff[p];
+                e.qsrc = (const unsigned char *)a.qseq + a.qoff[p];
+                e.rsrc = (const unsigned char *)a.rseq + a.roff[p];
+                e.qc = a.qcons + co;
+                e.rc = a.rcons + co;
+                e.mv = moves + (int64_t)p * move_words;
             }
         }
-        // the bases into LDS, from the aligned dwords holding them (an aligned
-        // dword holding a sequence byte lies within that byte's page)
-        const int qm = (int)((uintptr_t)qsrc & 3), rm = (int)((uintptr_t)rsrc & 3);
-        const int nqw = nq > 0 ? (qm + nq + 3) >> 2 : 0, nrw = nr > 0 ? (rm + nr + 3) >> 2 : 0;
-        for (int u = sl; u < nqw; u += 16)
-            lq4[slot][u] = reinterpret_cast<const uint32_t *>(qsrc - qm)[u];
-        for (int u = sl; u < nrw; u += 16)
-            lr4[slot][u] = reinterpret_cast<const uint32_t *>(rsrc - rm)[u];
-        wave_lds_sync();
-        const unsigned char *lq = (const unsigned char *)lq4[slot] + qm;
-        const unsigned char *lr = (const unsigned char *)lr4[slot] + rm;
-        char *qc = a.qcons + co, *rc = a.rcons + co;
-        const bool aligned = (co & 3) == 0;
-        int ci = row, cj = col;
-        const int nchunk = (len + 255) >> 8;
-        for (int c = 0; c < nchunk; ++c) {
-            const int k0 = c * 256 + sl * 16;
-            const int cnt = max(0, min(16, len - k0));
-            const uint32_t w = cnt > 0 ? moves[(int64_t)p * move_words + (k0 >> 4)] : 0u;
-            // row steps: every move but left (2); column steps: every move but
-            // up (1); ops are 0, 1 or 2
-            const uint32_t lo2 = cnt == 16 ? 0x55555555u : ((1u << (2 * cnt)) - 1u) & 0x55555555u;
-            const uint32_t hib = (w >> 1) & lo2, lob = w & lo2;
-            const int di = cnt - __builtin_popcount(hib & ~lob);
-            const int dj = cnt - __builtin_popcount(lob & ~hib);
-            int ei = di, ej = dj;  // inclusive scans over the row's 16 lanes
-#pragma unroll
-            for (int d = 1; d < 16; d <<= 1) {
-                const int oi = __shfl_up(ei, d, 16), oj = __shfl_up(ej, d, 16);
-                if (sl >= d) {
-                    ei += oi;
-                    ej += oj;
-                }
-            }
-            int xi = ci - (ei - di), xj = cj - (ej - dj);
-            uint32_t oq[4] = {0u, 0u, 0u, 0u}, orr[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                if (k < cnt) {
-                    const int op = (w >> (2 * k)) & 3;
-                    const uint32_t qb = op == 2 ? (uint32_t)'-' : (uint32_t)lq[xi - 1];
-                    const uint32_t rb = op == 1 ? (uint32_t)'-' : (uint32_t)lr[xj - 1];
-                    oq[k >> 2] |= qb << (8 * (k & 3));
-                    orr[k >> 2] |= rb << (8 * (k & 3));
-                    xi -= op != 2;
-                    xj -= op != 1;
-                }
-            }
-            if (cnt == 16 && aligned) {
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    reinterpret_cast<uint32_t *>(qc + k0)[d] = oq[d];
-                    reinterpret_cast<uint32_t *>(rc + k0)[d] = orr[d];
-                }
-            } else {
-                for (int k = 0; k < cnt; ++k) {
-                    qc[k0 + k] = (char)(oq[k >> 2] >> (8 * (k & 3)));
-                    rc[k0 + k] = (char)(orr[k >> 2] >> (8 * (k & 3)));
-                }
-            }
-            ci -= __shfl(ei, 15, 16);
-            cj -= __shfl(ej, 15, 16);
-        }
-        if (p >= 0 && sl == 0) {
-            qc[len] = 0;
-            rc[len] = 0;
-        }
-        wave_lds_sync();  // before the next pairs overwrite this wave's buffers
+        expand_row(e, lq4[slot], lr4[slot], sl);
     }
+}
+
+// Plain path, after sw_plain_kernel: one lane per pair walks the pair's
+// trace from its best cell (res: score, row, col, columns) and stores the
+// path as 2-bit moves plus the drop-in's score, start, end and length.
+__global__ void __launch_bounds__(256) sw_plain_walk_kernel(const uint32_t *trace, int64_t wpp,
+                                                            int R, int W, int qlen, int npairs,
+                                                            int pair0, const int4 *res,
+                                                            uint32_t *moves, int move_words,
+                                                            PlainOut out) {
+    const int p = blockIdx.x * 256 + (int)threadIdx.x;
+    if (p >= npairs) return;
+    const int4 r = res[p];
+    uint32_t *mv = moves + (int64_t)p * move_words;
+    uint32_t word = 0;
+    auto emit = [&](int k, int op, int, int) {
+        word |= (uint32_t)op << (2 * (k & 15));
+        if ((k & 15) == 15) {
+            mv[k >> 4] = word;
+            word = 0;
+        }
+    };
+    const PairTrace pt{(int64_t)p * wpp, 0, (int16_t)R, (int8_t)W, 0};
+    WalkOut wo = walk_cells<TraceReaderLean>(trace, r.x, r.y, r.z, pt, qlen, r.w, emit);
+    if (wo.len & 15) mv[wo.len >> 4] = word;
+    const int gp = pair0 + p;
+    out.score[gp] = (float)r.x;
+    out.start[gp] = wo.start;
+    out.end[gp] = wo.end;
+    out.alen[gp] = wo.len;
+}
+
+// Plain path: the consensus strings from the walks' moves, four pairs per
+// one-wave workgroup (dynamic LDS: 4 x (qw + rw) dwords).
+__global__ void __launch_bounds__(64) sw_plain_expand_kernel(const char *qseq, const char *rseq,
+                                                             int qlen, int rlen_max, int npairs,
+                                                             int pair0, const int4 *res,
+                                                             const uint32_t *moves,
+                                                             int move_words, int qw, int rw,
+                                                             PlainOut out) {
+    extern __shared__ uint32_t lds_x[];
+    const int slot = threadIdx.x >> 4, sl = threadIdx.x & 15;
+    const int p = blockIdx.x * 4 + slot;
+    ExpandPair e;
+    if (p < npairs) {
+        const int gp = pair0 + p;
+        const int4 r = res[p];
+        e.live = true;
+        e.len = out.alen[gp];
+        e.row = r.y;
+        e.col = r.z;
+        e.nq = min(qlen, r.y);
+        e.nr = min(r.w, r.z);
+        e.qsrc = (const unsigned char *)qseq + (int64_t)gp * (qlen + 1);
+        e.rsrc = (const unsigned char *)rseq + (int64_t)gp * (rlen_max + 1);
+        e.qc = out.qcons + (int64_t)gp * out.cstride;
+        e.rc = out.rcons + (int64_t)gp * out.cstride;
+        e.mv = moves + (int64_t)p * move_words;
+    }
+    expand_row(e, lds_x + slot * (qw + rw), lds_x + slot * (qw + rw) + qw, sl);
 }
 
 // Row classes: R per lane with 16-lane groups covers query length 16*R.
@@ -1167,10 +1238,10 @@ __global__ void __launch_bounds__(256) gather_words(Gather g) {
 template <int R>
 static hipError_t launch_plain_r(const char *q, const char *r, int qlen, int rlen_max,
                                  const KeyConsts &kc, int npairs, int pair0, uint32_t *trace,
-                                 int64_t wpp, const PlainOut &out, hipStream_t st) {
+                                 int64_t wpp, int4 *res, hipStream_t st) {
     size_t lds = sizeof(uint32_t) * (size_t)max(rlen_max, 1);
     hipLaunchKernelGGL(sw_plain_kernel<R>, dim3(npairs), dim3(64), lds, st, q, r, qlen, rlen_max,
-                       kc, npairs, pair0, trace, wpp, out);
+                       kc, npairs, pair0, trace, wpp, res);
     return hipGetLastError();
 }
 
@@ -1186,19 +1257,50 @@ int plain_rows_for(int qlen) {
 
 int plain_trace_words(int R) { return (R + 15) / 16; }
 
+// trace words per pair: whole 4-step granules of (rlen + 63) steps
+static int64_t plain_trace_stride(int qlen, int rlen) {
+    return (int64_t)((rlen + 63 + 3) & ~3) * plain_trace_words(plain_rows_for(qlen)) * 64;
+}
+
+// 2-bit moves per pair: a path has at most qlen + rlen moves
+static int plain_move_words(int qlen, int rlen) { return (qlen + rlen) / 16 + 1; }
+
+int64_t plain_scratch_words(int qlen, int rlen) {
+    return plain_trace_stride(qlen, rlen) + 4 + plain_move_words(qlen, rlen);
+}
+
 hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
-                        int npairs, int pair0, uint32_t *trace, int64_t wpp, const PlainOut &out,
+                        int npairs, int pair0, uint32_t *scratch, const PlainOut &out,
                         hipStream_t st) {
+    if (npairs <= 0) return hipSuccess;
     KeyConsts kc = make_consts(sc, kBytes);
-    switch (plain_rows_for(qlen)) {
+    const int R = plain_rows_for(qlen);
+    const int64_t wpp = plain_trace_stride(qlen, rlen_max);
+    const int mw = plain_move_words(qlen, rlen_max);
+    // [npairs x wpp trace][npairs int4 best cells][npairs x mw moves]; wpp is
+    // a multiple of 4 words, so the int4s are 16-byte aligned
+    uint32_t *trace = scratch;
+    int4 *res = reinterpret_cast<int4 *>(scratch + (int64_t)npairs * wpp);
+    uint32_t *moves = scratch + (int64_t)npairs * (wpp + 4);
+    hipError_t e;
+    switch (R) {
 #define SWIFT_PCASE(RR) \
-    case RR: return launch_plain_r<RR>(q, r, qlen, rlen_max, kc, npairs, pair0, trace, wpp, out, st);
+    case RR: e = launch_plain_r<RR>(q, r, qlen, rlen_max, kc, npairs, pair0, trace, wpp, res, st); \
+        break;
         SWIFT_PCASE(1) SWIFT_PCASE(2) SWIFT_PCASE(3) SWIFT_PCASE(4) SWIFT_PCASE(5)
         SWIFT_PCASE(6) SWIFT_PCASE(7) SWIFT_PCASE(8) SWIFT_PCASE(10) SWIFT_PCASE(12)
         SWIFT_PCASE(16)
 #undef SWIFT_PCASE
     default: return hipErrorInvalidValue;
     }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sw_plain_walk_kernel, dim3((npairs + 255) / 256), dim3(256), 0, st, trace,
+                       wpp, R, plain_trace_words(R), qlen, npairs, pair0, res, moves, mw, out);
+    const int qw = (qlen + 6) / 4, rw = (rlen_max + 6) / 4;
+    hipLaunchKernelGGL(sw_plain_expand_kernel, dim3((npairs + 3) / 4), dim3(64),
+                       sizeof(uint32_t) * 4 * (size_t)(qw + rw), st, q, r, qlen, rlen_max, npairs,
+                       pair0, res, moves, mw, qw, rw, out);
+    return hipGetLastError();
 }
 
 

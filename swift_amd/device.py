@@ -57,6 +57,34 @@ class DeviceBuffer:
         return C.c_void_p(self.ptr)
 
 
+class Stream:
+    """A non-blocking HIP stream of the library (for work that overlaps other
+    streams' work, such as pipeline graphs launched in turn)."""
+
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().swift_stream_create(C.byref(p)))
+        self.ptr = p.value
+
+    def synchronize(self) -> None:
+        check(lib().swift_stream_synchronize(C.c_void_p(self.ptr)))
+
+    def close(self) -> None:
+        if self.ptr:
+            check(lib().swift_stream_destroy(C.c_void_p(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def cptr(self):
+        return C.c_void_p(self.ptr)
+
+
 def synchronize() -> None:
     check(lib().swift_device_synchronize())
 

@@ -269,12 +269,17 @@ class PipelineGraph:
     def launch(self, stream=None) -> None:
         check(lib().swift_pipeline_graph_launch(self._h, stream))
 
+    def check(self) -> None:
+        """Length and capacity errors of the last replay (after its stream
+        has been synchronised)."""
+        check(lib().swift_pipeline_graph_check(self._h))
+
     def run(self, reads, strings=True) -> "Records":
         from .device import synchronize
         self.load(reads)
         self.launch()
         synchronize()
-        check(lib().swift_pipeline_graph_check(self._h))
+        self.check()
         return self.out.results(strings)
 
     def close(self):

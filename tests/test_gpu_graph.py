@@ -38,3 +38,34 @@ def test_graph_replays_match_eager(orc):
             assert_records_match(got, orc.pipeline(oix, reads, 3), rep)
     finally:
         g.close()
+
+
+def test_two_graphs_on_two_streams_match_oracle(orc):
+    """Two pipeline graphs, each over its own read and record buffers,
+    launched in turn on two streams so that their replays overlap on the
+    device: every replay's records equal the oracle's for its reads."""
+    from swift_amd import search, synth
+    from swift_amd.device import Stream
+    rng = np.random.default_rng(78)
+    seqs = [synth.random_bases(rng, n, 0.41) for n in (200_000, 150_000)]
+    ix = search.Index.from_sequences([s.tobytes() for s in seqs], 12, 1000)
+    oix = orc.Index([s.tobytes() for s in seqs], 12, 1000)
+    graphs = [search.PipelineGraph(ix, 400, 150, 3) for _ in range(2)]
+    streams = [Stream() for _ in range(2)]
+    try:
+        for rnd in range(3):
+            sets = []
+            for gi, g in enumerate(graphs):
+                reads, _ = synth.sample_reads(rng, seqs, 400, 150, 0.01, 0.001)
+                g.load(reads)
+                g.launch(streams[gi].cptr)
+                sets.append(reads)
+            for gi, g in enumerate(graphs):
+                streams[gi].synchronize()
+                g.check()
+                got = g.out.results(True)
+                assert got.n > 0
+                assert_records_match(got, orc.pipeline(oix, sets[gi], 3), (rnd, gi))
+    finally:
+        for g in graphs:
+            g.close()

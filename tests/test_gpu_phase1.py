@@ -184,7 +184,9 @@ def filter_tier_reads():
     a true locus of about 25 hits among singleton clusters; (b) random reads
     on a k=8 index of 1 Mbp, a few hundred hits that are all clusters of one;
     (c) a 24-base motif at 100 positions on the k=12 tuple grid: 100
-    clusters tied at two hits, more than one wave lists."""
+    clusters tied at two hits, more than one wave lists; (d) 100-base reads
+    on a k=6 index of 1 Mbp, about 40 entries per bucket and 3,800 hits per
+    strand."""
     rnd = random.Random(123)
     out = []
     g6 = bytes(rnd.choice(b"ACGT") for _ in range(300_000))
@@ -198,6 +200,10 @@ def filter_tier_reads():
     reads = [bytes(rnd.choice(b"ACGT") for _ in range(30)) + motif +
              bytes(rnd.choice(b"ACGT") for _ in range(40)) for _ in range(6)]
     out.append(([gm], 12, reads))
+    g6 = bytes(rnd.choice(b"ACGT") for _ in range(1_000_000))
+    reads = [g6[p:p + 100] for p in (rnd.randrange(0, 999_000) for _ in range(30))]
+    reads += [bytes(rnd.choice(b"ACGT") for _ in range(100)) for _ in range(5)]
+    out.append(([g6], 6, reads))
     return out
 
 

@@ -59,8 +59,9 @@ def _run(fn, qs, rs, q, r, extra):
 
 def test_noalloc_matches_allocating_entry_point():
     rnd = random.Random(77)
-    # 100 x 300 pairs: 23,296 trace words each, 720 per chunk of the 64 MiB
-    # array, so 3,000 pairs take 5 chunks; then fewer, longer pairs
+    # 100 x 300 pairs: 23,326 words each (trace, best cell, moves), 719 per
+    # chunk of the 64 MiB array, so 3,000 pairs take 5 chunks; then fewer,
+    # longer pairs
     for n, q, r in ((3000, 100, 300), (300, 150, 300), (40, 64, 120)):
         qs, rs = _pairs(rnd, n, q, r)
         want = _run("swift_align_sequences", qs, rs, q, r, [None])
