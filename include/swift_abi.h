@@ -113,9 +113,9 @@ int swift_align_sequences(char *querySeq_d, char *refSeq_d, int querySeqLength, 
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, limits and results as swift_align_sequences;
- * the traceback scratch is a fixed 64 MiB device array per GPU (held until
- * the process ends) instead of library-allocated memory, and the pairs run
- * in chunks that fit it.  SWIFT_ERR_CAPACITY when one pair's traceback
+ * the traceback scratch (trace, best cells, walk moves) is a fixed 64 MiB
+ * device array per GPU (held until the process ends) instead of
+ * library-allocated memory, and the pairs run in chunks that fit it.  SWIFT_ERR_CAPACITY when one pair's traceback
  * would not fit (not reachable within the plain kernel's length limits).
  * Every call on one GPU shares that GPU's array: calls whose kernels
  * overlap on one GPU race, so issue them on one stream or wait for one
