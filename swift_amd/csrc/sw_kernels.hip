@@ -256,8 +256,10 @@ struct Sweep {
                 el[r] = as_u(__builtin_elementwise_sub_sat(
                     as_us(h4), as_us(__builtin_amdgcn_perm(pen_l_hi, pen_l_lo, sel))));
             } else {
-                eu = vmad<PK>(opaque(vbit<PK>(4u, c)), c_de, vadd<PK>(h4, c_eo2));
-                el[r] = vmad<PK>(opaque(vbit<PK>(2u, c)), c_de, vadd<PK>(h4, c_eo1));
+                // int32: a select of the extension delta (a 32-bit multiply
+                // by the 0/1 flag would be a quarter-rate v_mul_lo_u32)
+                eu = h4 + c_eo2 + (c == 2u ? c_de : 0u);
+                el[r] = h4 + c_eo1 + (c == 1u ? c_de : 0u);
             }
             if constexpr (PK) {
                 static_assert(!(PK && MASKED), "packed steps are never masked");
