@@ -297,10 +297,11 @@ def c1_leg(steps, cpu):
 def c2_leg(n_reads, steps, cpu_reads):
     """Phase 1 + phase 2 over the C2 workload: 50 Mbp / 24 sequences, k=12,
     tuple threshold 1000, maxHits 3.  Reports the device index build time,
-    phase-1 reads/s (both strands mapped) against the HBM roofline, and
-    reads/s of the whole device pipeline (map -> 300-bp windows -> batched
-    SW -> best-hit choice) against its roofline, with the oracle's CPU
-    pipeline timed and compared on a sample."""
+    phase-1 reads/s (both strands mapped) against the HBM roofline, reads/s
+    of the whole device pipeline (map -> 300-bp windows -> batched SW ->
+    best-hit choice) against its roofline, phase 2 through the plain
+    per-pair path (c2_plain_phase2), and the oracle's CPU pipeline timed and
+    compared on a sample."""
     from swift_amd import search, synth
     from swift_amd._lib import check, lib
     from swift_amd.device import DeviceBuffer, synchronize
@@ -341,7 +342,7 @@ def c2_leg(n_reads, steps, cpu_reads):
         ix.align_batch(rb, out, mh)
     synchronize()
     pipe_s = (time.perf_counter() - t0) / steps
-    recs = out.results(strings=False)
+    recs = out.results(strings=True)
     res = {"workload": "C2: 50 Mbp / 24 sequences (GC 41 %, 5 % tandem repeats), "
                        f"{len(reads)} reads of 100 bp (5 % random), k=12, thr 1000, maxHits 3",
            "index_build_s": round(build_s, 3), "kept_tuples": info["total"],
@@ -357,6 +358,8 @@ def c2_leg(n_reads, steps, cpu_reads):
                                                                   [len(x) for x in seqs]),
                                                   pipe_s),
            "hits_per_strand": round(float(hits.mean()), 2)}
+    res["plain_phase2"] = c2_plain_phase2(reads, seqs, recs, map_s, steps)
+    log(f"[bench] C2 plain phase 2: {res['plain_phase2']}")
     if cpu_reads > 0:
         from oracle import oracle as O
         # the first reads and the fully random 5 % at the end
@@ -367,6 +370,93 @@ def c2_leg(n_reads, steps, cpu_reads):
             oix, sample, mh, f"first {cpu_reads - nrand} C2 reads + {nrand} random reads")
         res["parity_vs_oracle"] = pipeline_parity(ix, sample, want, mh)
     return res
+
+
+def c2_plain_phase2(reads, seqs, recs, phase1_s, steps):
+    """C2's phase 2 through the plain per-pair path, as the config names it:
+    every pipeline record's (query, window) pair -- the read or its reverse
+    complement (A->T, C->G, G->C, any other byte -> A), and up to 300 bases
+    from max(0, refPos - 100) with leading N stripped, else trailing N --
+    through swift_align_sequences, one call per read length (stride qlen + 1
+    and 301, NUL padded), inputs resident in HBM.  Reports the plain path's
+    GCUPS, the pipeline rate with it (phase-1 time + plain SW time; the
+    windows are cut on the host here, untimed), and whether its scores,
+    coordinates and strings equal the batched pipeline's records."""
+    from swift_amd._lib import check, lib
+    from swift_amd.device import DeviceBuffer, synchronize
+    n = recs.n
+    c = recs.cols
+    lut = np.full(256, ord("A"), np.uint8)
+    lut[ord("A")], lut[ord("C")], lut[ord("G")] = ord("T"), ord("G"), ord("C")
+    rlen_max = 300
+    win = np.zeros((n, rlen_max + 1), np.uint8)
+    wl = np.zeros(n, np.int64)
+    arrs = [np.frombuffer(x, np.uint8) for x in seqs]
+    for i in range(n):
+        sq = arrs[c["ref_idx"][i]]
+        rd = int(c["ref_distance"][i])
+        w = sq[rd:rd + rlen_max]
+        if len(w) and (w[0] in (78, 110) or w[-1] in (78, 110)):  # N / n at an end
+            isn = (w == 78) | (w == 110)
+            lead = int(np.argmin(isn)) if not isn.all() else len(w)
+            if lead:
+                w = w[lead:]
+            else:
+                keep = len(w) - int(np.argmin(isn[::-1]))
+                w = w[:keep]
+        win[i, :len(w)] = w
+        wl[i] = len(w)
+    qlens = np.array([len(reads[r]) for r in c["read"]], np.int64)
+    groups, cells = [], 0.0
+    for q in np.unique(qlens):
+        idx = np.nonzero(qlens == q)[0]
+        q = int(q)
+        qa = np.zeros((len(idx), q + 1), np.uint8)
+        for k, i in enumerate(idx):
+            rdx = np.frombuffer(reads[c["read"][i]], np.uint8)
+            qa[k, :q] = lut[rdx[::-1]] if c["strand"][i] else rdx
+        cl = q + rlen_max + 2
+        m = len(idx)
+        b = dict(q=DeviceBuffer.from_array(qa), r=DeviceBuffer.from_array(win[idx]),
+                 sc=DeviceBuffer(4 * m), st=DeviceBuffer(4 * m), en=DeviceBuffer(4 * m),
+                 al=DeviceBuffer(4 * m), qc=DeviceBuffer(m * cl), rc=DeviceBuffer(m * cl))
+        groups.append((q, idx, m, cl, b))
+        cells += float(q * wl[idx].sum())
+
+    def run():
+        for q, idx, m, cl, b in groups:
+            check(lib().swift_align_sequences(
+                C.c_void_p(b["q"].ptr), C.c_void_p(b["r"].ptr), q, rlen_max, C.c_float(2),
+                C.c_float(-1), C.c_float(-10), C.c_float(-1), C.c_void_p(b["sc"].ptr),
+                C.c_void_p(b["st"].ptr), C.c_void_p(b["en"].ptr), C.c_void_p(b["al"].ptr),
+                C.c_void_p(b["qc"].ptr), C.c_void_p(b["rc"].ptr), m, None))
+
+    run()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    synchronize()
+    sw_s = (time.perf_counter() - t0) / steps
+    same, sdiff = True, 0
+    for q, idx, m, cl, b in groups:
+        sc = b["sc"].download(np.float32, m).astype(np.int64)
+        st, en, al = (b[k].download(np.int32, m) for k in ("st", "en", "al"))
+        same &= bool(np.array_equal(sc, np.asarray(c["score"])[idx]) and
+                     np.array_equal(st, np.asarray(c["start"])[idx]) and
+                     np.array_equal(en, np.asarray(c["end"])[idx]) and
+                     np.array_equal(al, np.asarray(c["alen"])[idx]))
+        qc = b["qc"].download(np.uint8, m * cl)
+        rc = b["rc"].download(np.uint8, m * cl)
+        for k, i in enumerate(idx):
+            a = int(al[k])
+            if (qc[k * cl:k * cl + a].tobytes() != recs.qry_cons[i] or
+                    rc[k * cl:k * cl + a].tobytes() != recs.ref_cons[i]):
+                sdiff += 1
+    return {"pairs": n, "calls": len(groups), "cells": cells,
+            "gcups": round(cells / sw_s / 1e9, 2), "sw_ms": round(sw_s * 1e3, 3),
+            "pipeline_reads_per_s": round(len(reads) / (phase1_s + sw_s), 1),
+            "identical_to_batched": {"fields": same, "strings": sdiff == 0}}
 
 
 def c5_leg(index, g, off, lens, n_requests, seed=20261205, p1_per_kmer_read=None):

@@ -401,10 +401,13 @@ struct TraceReaderLean {
         const int t = (j - 1) + g;
         const int w = k >> kl;  // rows per word: 8 (packed halves) or 16
         const int bit = bitb + 2 * (k & ((1 << kl) - 1));
-        const int gi = ((t >> 2) * W + w) * 64 + g;
-        if (gi != cached) {
-            gran = tr[gi];
-            cached = gi;
+        // 32-bit granule index (a 24-bit multiply: t < 2^12, W <= 3); the
+        // byte offset is formed once, in 64 bits, at the load
+        const uint32_t gi = ((__umul24((uint32_t)(t >> 2), (uint32_t)W) + w) << 6) + g;
+        if ((int)gi != cached) {
+            gran = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(tr) +
+                                                     ((size_t)gi << 4));
+            cached = (int)gi;
         }
         const uint32_t lo = (t & 1) ? gran.y : gran.x, hi = (t & 1) ? gran.w : gran.z;
         const uint32_t word = (t & 2) ? hi : lo;
