@@ -170,6 +170,11 @@ int swift_get_tuning(int *waves_per_cu, int *allow_int16, int *walk_deferred);
  * inside graph capture and on serving slots, else 0.  No counterpart in the
  * reference; results are identical in every mode. */
 int swift_set_coarse_rows(int mode);
+
+/* Pairs per chunk of a swift_align_sequences call (0, the default: as many
+ * as 4 GiB of traceback scratch holds; the chunks run one after another).
+ * No counterpart in the reference; results are identical for every size. */
+int swift_set_plain_chunk(int max_pairs);
 int swift_get_coarse_rows(int *mode);
 
 /* Autotuner of the batched path: times every candidate setting (waves per

@@ -27,7 +27,8 @@ def lib() -> C.CDLL:
                      "swift_memcpy_dtoh", "swift_memset_device", "swift_device_synchronize",
                      "swift_device_count", "swift_sw_batch", "swift_align_sequences",
                      "swift_sw_batch_stats", "swift_set_tuning", "swift_set_coarse_rows",
-                     "swift_get_coarse_rows", "swift_set_map_mode", "swift_get_map_mode"):
+                     "swift_get_coarse_rows", "swift_set_map_mode", "swift_get_map_mode",
+                     "swift_set_plain_chunk"):
             getattr(L, name).restype = C.c_int
         L.swift_device_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.swift_device_free.argtypes = [C.c_void_p]

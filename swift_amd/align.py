@@ -226,6 +226,12 @@ def set_coarse_rows(mode: int) -> None:
     check(lib().swift_set_coarse_rows(int(mode)))
 
 
+def set_plain_chunk(max_pairs: int) -> None:
+    """Pairs per chunk of the plain path (0: by its 4 GiB scratch budget);
+    the chunks of a call run one after another."""
+    check(lib().swift_set_plain_chunk(int(max_pairs)))
+
+
 def get_coarse_rows() -> int:
     m = C.c_int()
     check(lib().swift_get_coarse_rows(C.byref(m)))

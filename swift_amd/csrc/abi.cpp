@@ -155,4 +155,13 @@ int swift_get_coarse_rows(int *mode) {
     return kOk;
 }
 
+int swift_set_plain_chunk(int max_pairs) {
+    if (max_pairs < 0) {
+        set_error("plain chunk size must be >= 0");
+        return kErrInvalid;
+    }
+    set_plain_chunk_pairs(max_pairs);
+    return kOk;
+}
+
 }  // extern "C"

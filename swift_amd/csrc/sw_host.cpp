@@ -70,8 +70,8 @@ void set_batch_timing(bool on) { g_timing.on = on; }
 const float *batch_timing_ms() { return g_timing.ms; }
 
 struct Workspace {
-    // plain path
-    DevBuf p_trace, p_res, p_ptr;
+    // plain path: scratch of a chunk (trace, best cells, moves)
+    DevBuf p_trace;
     // Marks the end of the last plain call's kernels on its stream.  Every
     // call writes and walks p_trace; a call on another stream waits for this
     // event first, so kernels of calls on different streams never share the
@@ -107,7 +107,7 @@ struct Workspace {
     int waves_per_cu = -1;    // persistent waves per CU for this workspace; -1: tuning()'s
     std::mutex mu;            // one batched run at a time per workspace
 
-    DevBuf *all_bufs[18] = {&p_trace, &p_res, &p_ptr, &trace, &res, &ptr, &bin_of, &order,
+    DevBuf *all_bufs[18] = {&p_trace, &trace, &res, &ptr, &bin_of, &order,
                             &order2, &tasks, &task_words, &scan_sums, &moves, &bins, &segs,
                             &counters, &totals, nullptr};
 };
@@ -232,7 +232,10 @@ int scoring_from_floats(float match, float mismatch, float gap_open, float gap_e
 // ---------------------------------------------------------------------------
 // Plain path (alignSequences drop-in).
 // ---------------------------------------------------------------------------
-static size_t g_plain_budget_words = (size_t)1 << 28;  // 1 GiB of trace, moves and best cells
+// scratch of a plain chunk: 4 GiB of trace, moves and best cells (a chunk's
+// walk is a latency-bound tail between sweeps, so fewer, larger chunks)
+static size_t g_plain_budget_words = (size_t)1 << 30;
+static int g_plain_chunk_pairs = 0;                     // > 0: at most this many pairs a chunk
 
 int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match, float mismatch,
                            float gap_open, float gap_ext, float *score_d, int *start_d,
@@ -261,19 +264,25 @@ int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match
     Workspace &w = ws();
     const int64_t wpp = plain_scratch_words(qlen, rlen);
     int chunk = (int)std::min<int64_t>(npairs, std::max<int64_t>(1, g_plain_budget_words / wpp));
+    if (g_plain_chunk_pairs > 0) chunk = std::min(chunk, g_plain_chunk_pairs);
+    const size_t need = (size_t)chunk * wpp * 4;
     if (!w.p_done) SWIFT_HIP_TRY(hipEventCreateWithFlags(&w.p_done, hipEventDisableTiming));
     if (w.p_pending) {
         // a larger trace replaces the buffer the last call's kernels may
         // still use; otherwise this stream queues behind them
-        if ((size_t)chunk * wpp * 4 > w.p_trace.bytes)
+        if (need > w.p_trace.bytes)
             SWIFT_HIP_TRY(hipEventSynchronize(w.p_done));
         else
             SWIFT_HIP_TRY(hipStreamWaitEvent(st, w.p_done, 0));
     }
-    if ((rc = w.p_trace.ensure((size_t)chunk * wpp * 4))) return rc;
+    if ((rc = w.p_trace.ensure(need))) return rc;
+    // The chunks run one after another on the caller's stream.  (Walking
+    // chunk c on a second stream while chunk c + 1 sweeps was slower: the
+    // walk's latency-bound waves took CU slots the sweep's single round of
+    // waves needed, and both kernels stretched.)
+    const PlainOut out{score_d, start_d, end_d, alen_d, qcons_d, rcons_d, (int64_t)qlen + rlen + 2};
     for (int c0 = 0; c0 < npairs; c0 += chunk) {
-        int cn = std::min(chunk, npairs - c0);
-        PlainOut out{score_d, start_d, end_d, alen_d, qcons_d, rcons_d, (int64_t)qlen + rlen + 2};
+        const int cn = std::min(chunk, npairs - c0);
         SWIFT_HIP_TRY(launch_plain(q_d, r_d, qlen, rlen, sc, cn, c0, w.p_trace.as<uint32_t>(),
                                    out, st));
     }
@@ -282,7 +291,7 @@ int align_sequences_status(char *q_d, char *r_d, int qlen, int rlen, float match
     return kOk;
 }
 
-void set_plain_budget_words(size_t words) { g_plain_budget_words = std::max<size_t>(words, 1); }
+void set_plain_chunk_pairs(int pairs) { g_plain_chunk_pairs = std::max(pairs, 0); }
 
 // ---------------------------------------------------------------------------
 // Batched path.

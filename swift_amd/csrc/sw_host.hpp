@@ -19,9 +19,17 @@ struct PlainOut;
 hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
                         int npairs, int pair0, uint32_t *scratch, const PlainOut &out,
                         hipStream_t st);
+// its two halves: the sweep (trace and best cells into scratch), then the
+// walk and consensus expansion (which read them)
+hipError_t launch_plain_sweep(const char *q, const char *r, int qlen, int rlen_max,
+                              const Scoring &sc, int npairs, int pair0, uint32_t *scratch,
+                              hipStream_t st);
+hipError_t launch_plain_walk(const char *q, const char *r, int qlen, int rlen_max, int npairs,
+                             int pair0, uint32_t *scratch, const PlainOut &out, hipStream_t st);
 int plain_rows_for(int qlen);
 int plain_trace_words(int R);
 int64_t plain_scratch_words(int qlen, int rlen);  // device words per pair
+void set_plain_chunk_pairs(int pairs);             // 0: chunks by the scratch budget alone
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st);
 
 // up to four device arrays of 32-bit words -> one host staging block

@@ -1274,38 +1274,60 @@ int64_t plain_scratch_words(int qlen, int rlen) {
     return plain_trace_stride(qlen, rlen) + 4 + plain_move_words(qlen, rlen);
 }
 
-hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
-                        int npairs, int pair0, uint32_t *scratch, const PlainOut &out,
-                        hipStream_t st) {
+// scratch of a plain chunk: [npairs x wpp trace][npairs int4 best cells]
+// [npairs x mw moves]; wpp is a multiple of 4 words, so the int4s are
+// 16-byte aligned
+struct PlainScratch {
+    int64_t wpp;
+    int mw;
+    uint32_t *trace;
+    int4 *res;
+    uint32_t *moves;
+    PlainScratch(int qlen, int rlen_max, int npairs, uint32_t *scratch)
+        : wpp(plain_trace_stride(qlen, rlen_max)), mw(plain_move_words(qlen, rlen_max)),
+          trace(scratch), res(reinterpret_cast<int4 *>(scratch + (int64_t)npairs * wpp)),
+          moves(scratch + (int64_t)npairs * (wpp + 4)) {}
+};
+
+hipError_t launch_plain_sweep(const char *q, const char *r, int qlen, int rlen_max,
+                              const Scoring &sc, int npairs, int pair0, uint32_t *scratch,
+                              hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
     KeyConsts kc = make_consts(sc, kBytes);
-    const int R = plain_rows_for(qlen);
-    const int64_t wpp = plain_trace_stride(qlen, rlen_max);
-    const int mw = plain_move_words(qlen, rlen_max);
-    // [npairs x wpp trace][npairs int4 best cells][npairs x mw moves]; wpp is
-    // a multiple of 4 words, so the int4s are 16-byte aligned
-    uint32_t *trace = scratch;
-    int4 *res = reinterpret_cast<int4 *>(scratch + (int64_t)npairs * wpp);
-    uint32_t *moves = scratch + (int64_t)npairs * (wpp + 4);
-    hipError_t e;
-    switch (R) {
+    const PlainScratch ps(qlen, rlen_max, npairs, scratch);
+    switch (plain_rows_for(qlen)) {
 #define SWIFT_PCASE(RR) \
-    case RR: e = launch_plain_r<RR>(q, r, qlen, rlen_max, kc, npairs, pair0, trace, wpp, res, st); \
-        break;
+    case RR: return launch_plain_r<RR>(q, r, qlen, rlen_max, kc, npairs, pair0, ps.trace, ps.wpp, \
+                                       ps.res, st);
         SWIFT_PCASE(1) SWIFT_PCASE(2) SWIFT_PCASE(3) SWIFT_PCASE(4) SWIFT_PCASE(5)
         SWIFT_PCASE(6) SWIFT_PCASE(7) SWIFT_PCASE(8) SWIFT_PCASE(10) SWIFT_PCASE(12)
         SWIFT_PCASE(16)
 #undef SWIFT_PCASE
     default: return hipErrorInvalidValue;
     }
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(sw_plain_walk_kernel, dim3((npairs + 255) / 256), dim3(256), 0, st, trace,
-                       wpp, R, plain_trace_words(R), qlen, npairs, pair0, res, moves, mw, out);
+}
+
+hipError_t launch_plain_walk(const char *q, const char *r, int qlen, int rlen_max, int npairs,
+                             int pair0, uint32_t *scratch, const PlainOut &out, hipStream_t st) {
+    if (npairs <= 0) return hipSuccess;
+    const int R = plain_rows_for(qlen);
+    const PlainScratch ps(qlen, rlen_max, npairs, scratch);
+    hipLaunchKernelGGL(sw_plain_walk_kernel, dim3((npairs + 255) / 256), dim3(256), 0, st,
+                       ps.trace, ps.wpp, R, plain_trace_words(R), qlen, npairs, pair0, ps.res,
+                       ps.moves, ps.mw, out);
     const int qw = (qlen + 6) / 4, rw = (rlen_max + 6) / 4;
     hipLaunchKernelGGL(sw_plain_expand_kernel, dim3((npairs + 3) / 4), dim3(64),
                        sizeof(uint32_t) * 4 * (size_t)(qw + rw), st, q, r, qlen, rlen_max, npairs,
-                       pair0, res, moves, mw, qw, rw, out);
+                       pair0, ps.res, ps.moves, ps.mw, qw, rw, out);
     return hipGetLastError();
+}
+
+hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
+                        int npairs, int pair0, uint32_t *scratch, const PlainOut &out,
+                        hipStream_t st) {
+    hipError_t e = launch_plain_sweep(q, r, qlen, rlen_max, sc, npairs, pair0, scratch, st);
+    if (e != hipSuccess) return e;
+    return launch_plain_walk(q, r, qlen, rlen_max, npairs, pair0, scratch, out, st);
 }
 
 

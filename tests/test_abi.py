@@ -59,6 +59,9 @@ def test_mode_switches_validate_without_gpu(built):
             assert b"map mode" in L.swift_last_error()
             assert L.swift_set_coarse_rows(bad) != 0
             assert b"row class" in L.swift_last_error()
+        assert L.swift_set_plain_chunk(-1) != 0
+        assert b"chunk" in L.swift_last_error()
+        assert L.swift_set_plain_chunk(7) == 0 and L.swift_set_plain_chunk(0) == 0
         assert L.swift_set_tuning(65, 1, 1) != 0
         assert L.swift_set_tuning(-1, 1, 1) != 0
         assert L.swift_set_tuning(8, 1, 0) == 0

@@ -89,6 +89,31 @@ def test_plain_shapes_vs_oracle(A, orc, ql, rl):
         compare(res.record(i), orc.sw(rs[i], qs[i], ref_len=rl), (ql, rl, i))
 
 
+def test_plain_chunks_vs_oracle(A, orc):
+    """A call of many chunks (7 pairs each) through the one scratch buffer;
+    every pair against the oracle, twice in a row (the second call with
+    other shapes reuses the buffer)."""
+    rnd = random.Random(4242)
+    try:
+        A.set_plain_chunk(7)
+        for ql, rl, n in ((100, 300, 200), (64, 150, 61)):
+            qs, rs = [], []
+            for _ in range(n):
+                reg = bytes(rnd.choice(b"ACGT") for _ in range(rl))
+                o = rnd.randint(0, rl - ql)
+                q = bytearray(reg[o:o + ql])
+                for k in range(ql):
+                    if rnd.random() < 0.05:
+                        q[k] = rnd.choice(b"ACGTN")
+                qs.append(bytes(q))
+                rs.append(reg[:rnd.randint(rl // 2, rl)])
+            res = A.align_sequences(qs, rs, ql, rl)
+            for i in range(n):
+                compare(res.record(i), orc.sw(rs[i], qs[i], ref_len=rl), (ql, rl, i))
+    finally:
+        A.set_plain_chunk(0)
+
+
 def test_plain_exit_on_bad_score(A):
     with pytest.raises(A.__class__ if False else Exception):
         A.align_sequences([b"ACGT"], [b"ACGT"], scoring=A.Scoring(2.5, -1, -10, -1))
