@@ -500,6 +500,8 @@ __device__ __forceinline__ void wave_store_fence() {
 // smithWaterman_cpu), reference at refSeq + p*(rlen+1), columns stopping at
 // its first NUL.  LDS holds the reference codes of the pair.
 // ---------------------------------------------------------------------------
+constexpr int kPlainPad = 64;  // LDS words before column 0 of the plain kernel's reference
+
 template <int R>
 __global__ void __launch_bounds__(64)
 sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, int qlen, int rlen_max,
@@ -513,16 +515,20 @@ sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, in
     const char *qs = qseq + (int64_t)(pair0 + p) * (qlen + 1);
     const char *rs = rseq + (int64_t)(pair0 + p) * (rlen_max + 1);
 
-    // reference codes into LDS; the column count stops at the first NUL
+    // the column count stops at the first NUL
     int first_nul = rlen_max;
-    for (int j = g; j < rlen_max; j += 64) {
-        unsigned char c = (unsigned char)rs[j];
-        lds_rr[j] = byte_rcode(c);
-        if (c == 0) first_nul = min(first_nul, j);
-    }
+    for (int j = g; j < rlen_max; j += 64)
+        if (rs[j] == 0) first_nul = min(first_nul, j);
 #pragma unroll
     for (int s = 32; s >= 1; s >>= 1) first_nul = min(first_nul, __shfl_xor(first_nul, s, 64));
     const int rlen = first_nul;
+    // reference codes into LDS at kPlainPad + j, zero for every column a
+    // lane reaches outside [0, rlen) (j from -63 to rlen + 62), so each step
+    // reads its column without a branch
+    for (int x = g; x < rlen_max + 2 * kPlainPad; x += 64) {
+        const int j = x - kPlainPad;
+        lds_rr[x] = j >= 0 && j < rlen ? byte_rcode((unsigned char)rs[j]) : 0u;
+    }
     __syncthreads();
 
     S sw;
@@ -532,19 +538,28 @@ sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, in
         int row = g * R + r;
         sw.q[r] = row < qlen ? byte_qcode((unsigned char)qs[row]) : 0u;
     }
-    uint32_t *tr = trace + (int64_t)p * trace_words_per_pair;
+    // the pair's trace through a buffer resource: 32-bit byte offsets
+    // (a pair's trace is under 2^21 words)
+    const __amdgpu_buffer_rsrc_t tr_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        trace + (int64_t)p * trace_words_per_pair, 0, (int)0x7FFFFFFF, 0x00020000);
     const int T = rlen + 63;
     const int main_lo = 63, main_hi = rlen;  // every lane inside [0, rlen)
     auto run = [&](auto masked, int t) {
         int j = t - g;
-        uint32_t rr = (j >= 0 && j < rlen) ? lds_rr[j] : 0u;
-        uint32_t vmask = (j >= 0 && j < rlen) ? ~0u : 0u;
+        uint32_t rr = lds_rr[kPlainPad + j];
+        uint32_t vmask = 0;
+        if constexpr (decltype(masked)::value) vmask = (j >= 0 && j < rlen) ? ~0u : 0u;
         uint32_t a1[S::W];
 #pragma unroll
         for (int w = 0; w < S::W; ++w) a1[w] = 0;
         sw.template step<decltype(masked)::value>(rr, j, vmask, 0u, a1);
+        // byte offset of word (t >> 2, w, g, t & 3) of the granule layout
+        // (trace_word: 64 lanes x 4 steps = 1 KB per step group and word)
+        const uint32_t o = (uint32_t)(t >> 2) * (S::W * 1024u) + (uint32_t)g * 16u +
+                           (uint32_t)(t & 3) * 4u;
 #pragma unroll
-        for (int w = 0; w < S::W; ++w) tr[trace_word<S::W>(t, w, g)] = a1[w];
+        for (int w = 0; w < S::W; ++w)
+            __builtin_amdgcn_raw_buffer_store_b32(a1[w], tr_rsrc, o + (uint32_t)w * 1024u, 0, 0);
     };
     sweep_steps(T, min(main_lo, T), main_hi, run);
 
@@ -1244,7 +1259,7 @@ template <int R>
 static hipError_t launch_plain_r(const char *q, const char *r, int qlen, int rlen_max,
                                  const KeyConsts &kc, int npairs, int pair0, uint32_t *trace,
                                  int64_t wpp, int4 *res, hipStream_t st) {
-    size_t lds = sizeof(uint32_t) * (size_t)max(rlen_max, 1);
+    size_t lds = sizeof(uint32_t) * (size_t)(rlen_max + 2 * kPlainPad);
     hipLaunchKernelGGL(sw_plain_kernel<R>, dim3(npairs), dim3(64), lds, st, q, r, qlen, rlen_max,
                        kc, npairs, pair0, trace, wpp, res);
     return hipGetLastError();
