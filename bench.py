@@ -255,9 +255,10 @@ def oracle_pipeline_baseline(oix, reads, mh, what):
 def c1_leg(steps, cpu):
     """The small config (1 Mbp uniform reference, 1,000 reads of 100 bp at 2 %
     substitutions and 0.2 % indels, k=12, thr 1000, maxHits 3): reads/s of the
-    whole device pipeline with its roofline fraction, and the oracle's CPU
-    pipeline over the same reads timed beside it and compared record by
-    record."""
+    whole device pipeline with its roofline fraction, phase 2 through the
+    plain per-pair path (plain_phase2; phase 1 timed by swift_map_reads into
+    preallocated outputs), and the oracle's CPU pipeline over the same reads
+    timed beside it and compared record by record."""
     from swift_amd import search, synth
     from swift_amd.device import synchronize
     seqs, reads, _ = synth.c1_workload()
@@ -275,8 +276,20 @@ def c1_leg(steps, cpu):
         ix.align_batch(rb, out, mh)
     synchronize()
     pipe_s = (time.perf_counter() - t0) / reps
-    recs = out.results(strings=False)
+    recs = out.results(strings=True)
     hits = ix.map_batch(rb, mh)
+    # phase 1 alone into preallocated outputs (swift_map_reads synchronises)
+    from swift_amd._lib import check, lib
+    from swift_amd.device import DeviceBuffer
+    ns = 2 * rb.n
+    mbufs = [DeviceBuffer(ns * mh), DeviceBuffer(4 * ns * mh), DeviceBuffer(4 * ns * mh),
+             DeviceBuffer(4 * ns)]
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(lib().swift_map_reads(ix._h, rb.seq.cptr, rb.off.cptr, rb.len.cptr, rb.n,
+                                    rb.max_len, mh, *[b.cptr for b in mbufs], None, None,
+                                    None))
+    map_s = (time.perf_counter() - t0) / reps
     lens = [len(r) for r in reads]
     res = {"workload": "C1: 1 Mbp uniform reference, 1000 reads of 100 bp (2 % subs, 0.2 % "
                        "indels), k=12, thr 1000, maxHits 3",
@@ -285,7 +298,8 @@ def c1_leg(steps, cpu):
            "pipeline_ms": round(pipe_s * 1e3, 3), "records": recs.n,
            "roofline": pipeline_roofline(phase1_bytes(lens, hits.nhits, hits.nbest),
                                          record_cells(recs, lens, [len(s) for s in seqs]),
-                                         pipe_s)}
+                                         pipe_s),
+           "plain_phase2": plain_phase2(reads, seqs, recs, map_s, reps)}
     if cpu:
         from oracle import oracle as O
         oix = O.Index(seqs, 12, 1000)
@@ -300,7 +314,7 @@ def c2_leg(n_reads, steps, cpu_reads):
     phase-1 reads/s (both strands mapped) against the HBM roofline, reads/s
     of the whole device pipeline (map -> 300-bp windows -> batched SW ->
     best-hit choice) against its roofline, phase 2 through the plain
-    per-pair path (c2_plain_phase2), and the oracle's CPU pipeline timed and
+    per-pair path (plain_phase2), and the oracle's CPU pipeline timed and
     compared on a sample."""
     from swift_amd import search, synth
     from swift_amd._lib import check, lib
@@ -358,7 +372,7 @@ def c2_leg(n_reads, steps, cpu_reads):
                                                                   [len(x) for x in seqs]),
                                                   pipe_s),
            "hits_per_strand": round(float(hits.mean()), 2)}
-    res["plain_phase2"] = c2_plain_phase2(reads, seqs, recs, map_s, steps)
+    res["plain_phase2"] = plain_phase2(reads, seqs, recs, map_s, steps)
     log(f"[bench] C2 plain phase 2: {res['plain_phase2']}")
     if cpu_reads > 0:
         from oracle import oracle as O
@@ -372,8 +386,9 @@ def c2_leg(n_reads, steps, cpu_reads):
     return res
 
 
-def c2_plain_phase2(reads, seqs, recs, phase1_s, steps):
-    """C2's phase 2 through the plain per-pair path, as the config names it:
+def plain_phase2(reads, seqs, recs, phase1_s, steps):
+    """Phase 2 of C1 and C2 through the plain per-pair path, as the configs
+    name it:
     every pipeline record's (query, window) pair -- the read or its reverse
     complement (A->T, C->G, G->C, any other byte -> A), and up to 300 bases
     from max(0, refPos - 100) with leading N stripped, else trailing N --
