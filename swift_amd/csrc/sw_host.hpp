@@ -19,13 +19,6 @@ struct PlainOut;
 hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, const Scoring &sc,
                         int npairs, int pair0, uint32_t *scratch, const PlainOut &out,
                         hipStream_t st);
-// its two halves: the sweep (trace and best cells into scratch), then the
-// walk and consensus expansion (which read them)
-hipError_t launch_plain_sweep(const char *q, const char *r, int qlen, int rlen_max,
-                              const Scoring &sc, int npairs, int pair0, uint32_t *scratch,
-                              hipStream_t st);
-hipError_t launch_plain_walk(const char *q, const char *r, int qlen, int rlen_max, int npairs,
-                             int pair0, uint32_t *scratch, const PlainOut &out, hipStream_t st);
 int plain_rows_for(int qlen);
 int plain_trace_words(int R);
 int64_t plain_scratch_words(int qlen, int rlen);  // device words per pair

@@ -1304,9 +1304,10 @@ struct PlainScratch {
           moves(scratch + (int64_t)npairs * (wpp + 4)) {}
 };
 
-hipError_t launch_plain_sweep(const char *q, const char *r, int qlen, int rlen_max,
-                              const Scoring &sc, int npairs, int pair0, uint32_t *scratch,
-                              hipStream_t st) {
+// the sweep: trace and best cells into scratch
+static hipError_t launch_plain_sweep(const char *q, const char *r, int qlen, int rlen_max,
+                                     const Scoring &sc, int npairs, int pair0, uint32_t *scratch,
+                                     hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
     KeyConsts kc = make_consts(sc, kBytes);
     const PlainScratch ps(qlen, rlen_max, npairs, scratch);
@@ -1322,8 +1323,10 @@ hipError_t launch_plain_sweep(const char *q, const char *r, int qlen, int rlen_m
     }
 }
 
-hipError_t launch_plain_walk(const char *q, const char *r, int qlen, int rlen_max, int npairs,
-                             int pair0, uint32_t *scratch, const PlainOut &out, hipStream_t st) {
+// the walk and consensus expansion, which read them
+static hipError_t launch_plain_walk(const char *q, const char *r, int qlen, int rlen_max,
+                                    int npairs, int pair0, uint32_t *scratch, const PlainOut &out,
+                                    hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
     const int R = plain_rows_for(qlen);
     const PlainScratch ps(qlen, rlen_max, npairs, scratch);
