@@ -10,7 +10,8 @@ oracle/_ref is built:
 
 * C3: pairs from synth.c3_pairs_fast, the headline pairs' generator;
 * C2: the full 50 Mbp / 24-sequence genome of synth.c2_workload, with reads
-  including the fully random 5 %;
+  including the fully random 5 %, and its phase 2 also through the plain
+  per-pair path;
 * C4: the full 3.1 Gbp genome of synth.genome_c4 (built once for the
   module), 150-bp reads through the eager pipeline and the hipGraph;
 * C5: serving requests from synth.c5_requests (log-uniform 1..4096 reads,
@@ -74,8 +75,28 @@ def test_c3_fast_pairs_vs_oracle_and_reference(orc):
 
 
 # ------------------------------------------------------------------ C2
+def region(seq, ref_distance, width=300):
+    """The record's window as the pipeline cuts it (alignQueries_cpu_mt's
+    getRefSeq2, src/input.cu:348-403): up to `width` bases from
+    ref_distance, leading N/n stripped, else trailing N/n."""
+    w = seq[ref_distance:ref_distance + width]
+    lead = len(w) - len(w.lstrip(b"Nn"))
+    return w[lead:] if lead else w.rstrip(b"Nn")
+
+
+def revcomp(read):
+    """qryGetReverseComplement (src/query.cu:265-282): A->T, C->G, G->C,
+    every other byte -> A."""
+    m = {ord("A"): ord("T"), ord("C"): ord("G"), ord("G"): ord("C")}
+    return bytes(m.get(c, ord("A")) for c in reversed(read))
+
+
 def test_c2_workload_vs_oracle(orc):
-    from swift_amd import search, synth
+    """The C2 pipeline against the oracle's; then its phase 2 through the
+    plain per-pair path (the configuration's "plain phase-2"): every
+    record's (query, window) pair through alignSequences, one call per read
+    length, against the oracle's record."""
+    from swift_amd import align, search, synth
     seqs, reads, truth = synth.c2_workload(n_reads=2000)   # + 100 fully random reads
     assert sum(1 for t in truth if t[0] < 0) == 100
     ix = search.Index.from_sequences(seqs, 12, 1000)
@@ -83,6 +104,19 @@ def test_c2_workload_vs_oracle(orc):
     want = orc.pipeline_mt(orc.Index(seqs, 12, 1000), reads, 16)
     assert_records_match(got, want, "C2")
     assert got.n > 2 * 2000
+    by_len = {}
+    for i, w in enumerate(want):
+        r = reads[w["read"]]
+        q = revcomp(r) if w["strand"] else r
+        by_len.setdefault(len(q), []).append((i, q, region(seqs[w["refIdx"]], w["refDistance"])))
+    for ql, items in by_len.items():
+        res = align.align_sequences([q for _, q, _ in items], [x for _, _, x in items], ql, 300)
+        for k, (i, _, _) in enumerate(items):
+            g, w = res.record(k), want[i]
+            assert (g["score"], g["start"], g["end"], g["len"]) == \
+                (w["score"], w["start"], w["end"], w["len"]), ("C2 plain", i)
+            assert g["qry_cons"] == w["qry_cons"] and g["ref_cons"] == w["ref_cons"], \
+                ("C2 plain", i)
 
 
 # ------------------------------------------------------------------ C4 / C5
