@@ -529,6 +529,11 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     // follows its sweep on the same stream, overlapping the sweeps still
     // running on the others.  Without sizes every class kernel is launched
     // round-robin and the walk follows all sweeps.
+    // Per-class walks overlap the other classes' sweeps only when the classes
+    // spread over side streams.  With every class on the caller's stream
+    // (serving slots) one walk after the last class is cheaper: a walk kernel
+    // is a latency-bound tail, so one instead of one per class.
+    const bool class_walks = ba.walk_deferred && sync_sizes && w.nside > 0;
     auto launch_classes = [&](const TaskArgs &t, bool with16, int pass, const int32_t *seg_host,
                               const int64_t *steps_host) -> int {
         int live[2 * kRBins], nlive = 0;
@@ -576,7 +581,7 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             hipStream_t s2 = nside ? w.side[lane_of[seg]] : st;
             const int wpc = w.waves_per_cu >= 0 ? w.waves_per_cu : tuning().waves_per_cu;
             SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, wpc, s2));
-            if (ba.walk_deferred && seg_host) {  // per-class walk
+            if (class_walks && seg_host) {  // per-class walk
                 const int slots = cls == 0 ? kSlots16 : kSlots32;
                 const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;
                 SWIFT_HIP_TRY(launch_walk(b, b.order, t.seg_first + seg, t.seg_len + seg, maxp,
@@ -642,10 +647,11 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     w.stats.trace_words = words2;
 
     // ---- walk-back: fused into the forward kernels; or, deferred, after each
-    // class's sweep (host-known class sizes) or else here, over every binned
-    // pair of pass 1 (re-run pairs carry their pass-2 results by now)
+    // class's sweep (host-known class sizes, side streams) or else here, over
+    // every binned pair of pass 1 (re-run pairs carry their pass-2 results by
+    // now)
     SWIFT_HIP_TRY(mark(3, st, w.timed));
-    if (ba.walk_deferred && !sync_sizes) {
+    if (ba.walk_deferred && !class_walks) {
         ba.order = w.order.as<int32_t>();
         SWIFT_HIP_TRY(launch_walk(ba, w.order.as<int32_t>(), nullptr, cnt + 1, n,
                                   w.moves.as<uint32_t>(), kMoveWords, st));
