@@ -948,10 +948,10 @@ map_strands_hash(MapArgs a, const int32_t *list, const int32_t *list_count, int3
 // k-mer slot of hit e comes from an owner map (slot + 1 written at the
 // slot's first hit, then a workgroup max-scan) kept in bitmap A's words
 // before the bitmaps are needed.
-template <int THREADS, int CAP, int SMALL, int SMALLCAP, int MAXQ, int WPE>
+template <int THREADS, int CAP, int SMALL, int SMALLCAP, int MAXQ, int WPE, bool LIST>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
-map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
-                   int32_t *mid_count) {
+map_strands_filter(MapArgs a, const int32_t *src, const int32_t *src_count, int32_t *ovl,
+                   int32_t *ovl_count, int32_t *mid, int32_t *mid_count) {
     constexpr int NW = THREADS / 64;
     constexpr int PER = (MAXQ + THREADS) / THREADS;
     constexpr int NS = THREADS * PER;            // k-mer slots, s = thread * PER + u
@@ -960,13 +960,16 @@ map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
     constexpr int kWords = kBits / 32;
     constexpr uint32_t kEmpty = 0xFFFFFFFFu;
     constexpr int kCand = 64;
-    static_assert(CAP % THREADS == 0 && HPT % 2 == 0 && CAP / 2 <= 2 * kWords,
-                  "the owner map (16 bits per hit) fits bitmap A and B's words");
+    // the owner map (16 bits per hit) lives in bitmap A and B's words while
+    // it fits them, else in an array of its own
+    constexpr bool kOwnSep = CAP / 2 > 2 * kWords;
+    static_assert(CAP % THREADS == 0 && HPT % 2 == 0, "owner entries two to a word");
     static_assert(NS < 65535 && MAXQ < 1024 && HPT % 3 == 0,
                   "owner entries; three 10-bit query offsets per register");
     static_assert(4 * SMALLCAP <= 3 * SMALL && (SMALL & (SMALL - 1)) == 0,
                   "candidate clusters fill <= 3/4 of a power-of-two table");
     __shared__ __align__(16) uint32_t bm[2 * kWords];   // bitmaps A | B; first the owner map
+    __shared__ __align__(16) uint32_t own_sep[kOwnSep ? CAP / 2 : 1];
     __shared__ uint32_t skey[SMALL], scnt[SMALL], smin[SMALL];
     __shared__ uint32_t ccid[SMALLCAP];          // candidate hits: cluster id,
     __shared__ uint32_t cqc[SMALLCAP];           // offset | count returned by its insertion << 16
@@ -979,16 +982,21 @@ map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
     __shared__ uint32_t sh_best;
     __shared__ int32_t sh_ncand, sh_nhit;
     __shared__ uint32_t scid0[kMaxIndexSeqs];
-    const int n = 2 * a.rd.nreads;
+    // LIST: the strands of src (src_count of them), else every strand
+    const int n = LIST ? *src_count : 2 * a.rd.nreads;
+    auto strand_at = [&](int i) { return LIST ? src[i] : i; };
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     for (int i = t; i < 2 * kWords; i += THREADS) bm[i] = 0;
+    if constexpr (kOwnSep)
+        for (int i = t; i < CAP / 2; i += THREADS) own_sep[i] = 0;
     for (int i = t; i < SMALL; i += THREADS) {
         skey[i] = kEmpty;
         scnt[i] = 0;
         smin[i] = 0xFFFFFFFFu;
     }
     for (int i = t; i < a.ix.nseq; i += THREADS) scid0[i] = a.ix.cid0[i];
-    uint16_t *owner = reinterpret_cast<uint16_t *>(bm);
+    uint32_t *ow_words = kOwnSep ? own_sep : bm;  // the owner map, two entries a word
+    uint16_t *owner = reinterpret_cast<uint16_t *>(ow_words);
     // the 4-byte entries through a buffer resource: one 32-bit offset register
     // per load in flight (the index holds < 2^32 bytes of them when cluster
     // ids fit 32 bits; loads past the range return 0 instead of faulting)
@@ -1044,29 +1052,30 @@ map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
     int nx_len = 0;
     int64_t nx_off = 0;
     if ((int)blockIdx.x < n) {
-        const int s0 = blockIdx.x;
+        const int s0 = strand_at(blockIdx.x);
         unsigned char nb[PER];
         L0 = fetch(a.rd.len[s0 >> 1], a.rd.off[s0 >> 1], nb);
         store(s0, L0, nb);
         if ((int)blockIdx.x + G < n) {
-            const int r = (blockIdx.x + G) >> 1;
+            const int r = strand_at(blockIdx.x + G) >> 1;
             nx_len = a.rd.len[r];
             nx_off = a.rd.off[r];
         }
     }
     __syncthreads();
     heads(L0, c, st);
-    for (int sidx = blockIdx.x; sidx < n; sidx += G) {
-        const bool more = sidx + G < n;
-        const int nsidx = more ? sidx + G : 0;
+    for (int it = blockIdx.x; it < n; it += G) {
+        const int sidx = strand_at(it);
+        const bool more = it + G < n;
+        const int nsidx = more ? strand_at(it + G) : 0;
         unsigned char nb[PER];
         int nL = 0;
 #pragma unroll
         for (int u = 0; u < PER; ++u) nb[u] = 0;
         if (more) {
             nL = fetch(nx_len, nx_off, nb);
-            if (sidx + 2 * G < n) {
-                const int r = (sidx + 2 * G) >> 1;
+            if (it + 2 * G < n) {
+                const int r = strand_at(it + 2 * G) >> 1;
                 nx_len = a.rd.len[r];
                 nx_off = a.rd.off[r];
             }
@@ -1112,7 +1121,7 @@ map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
         uint32_t m = 0;
 #pragma unroll
         for (int j = 0; j < HPT / 2; ++j) {
-            ow[j] = bm[t * (HPT / 2) + j];
+            ow[j] = ow_words[t * (HPT / 2) + j];
             m = max(m, max(ow[j] & 0xFFFFu, ow[j] >> 16));
         }
         uint32_t wm = m;  // wave inclusive max-scan of the thread maxima
@@ -1134,7 +1143,7 @@ map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
                 const uint32_t lo = max(carry, ow[j] & 0xFFFFu);
                 const uint32_t hi = max(lo, ow[j] >> 16);
                 carry = hi;
-                bm[t * (HPT / 2) + j] = lo | (hi << 16);
+                ow_words[t * (HPT / 2) + j] = lo | (hi << 16);
             }
         }
         __syncthreads();
@@ -1163,7 +1172,7 @@ map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
         heads(nL, c, st);  // in flight through the rest of this strand
         if (t * HPT < total) {
 #pragma unroll
-            for (int j = 0; j < HPT / 2; ++j) bm[t * (HPT / 2) + j] = 0;
+            for (int j = 0; j < HPT / 2; ++j) ow_words[t * (HPT / 2) + j] = 0;
         }
         __syncthreads();
         // bitmap A: seen; bitmap B: seen again
@@ -1304,6 +1313,8 @@ map_strands_filter(MapArgs a, int32_t *ovl, int32_t *ovl_count, int32_t *mid,
         // clear the bitmaps and, when it was used, the table
 #pragma unroll
         for (int i = t; i < 2 * kWords; i += THREADS) bm[i] = 0;
+    if constexpr (kOwnSep)
+        for (int i = t; i < CAP / 2; i += THREADS) own_sep[i] = 0;
         if (ncand_all > 0 && ncand_all <= SMALLCAP) {
             for (int i = t; i < SMALL; i += THREADS) {
                 skey[i] = kEmpty;
@@ -1570,12 +1581,25 @@ hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
             // workgroups per CU), strands past its cap on to the full one
             // through the wave tier's (here unused) overflow list
             constexpr int kWpe = 8;  // waves per SIMD: kWpe / 2 workgroups of 8 waves per CU
-            hipLaunchKernelGGL((map_strands_filter<512, kHashCap, 1024, 768, kShortRead, kWpe>),
-                               dim3(kWpe / 2 * mid_blocks), dim3(512), 0, st, a, a.ovl_list,
-                               a.ovl_count, a.mid_list, a.mid_count);
+            hipLaunchKernelGGL(
+                (map_strands_filter<512, kHashCap, 1024, 768, kShortRead, kWpe, false>),
+                dim3(kWpe / 2 * mid_blocks), dim3(512), 0, st, a, nullptr, nullptr, a.ovl_list,
+                a.ovl_count, a.mid_list, a.mid_count);
             hipLaunchKernelGGL((map_strands_hash<512, kHashSlots, kHashCap, kShortRead, 4>),
                                dim3(2 * mid_blocks), dim3(512), 0, st, a, a.ovl_list, a.ovl_count,
                                a.mid_list, a.mid_count);
+            // strands past kHashCap hits (reads near the short-read limit):
+            // the filter stage at 1024 threads, one workgroup per CU; what it
+            // cannot take goes on to the workgroup sort
+            hipLaunchKernelGGL(
+                (map_strands_filter<1024, kLongFilterCap, 2048, 1536, kShortRead, 4, true>),
+                dim3(mid_blocks / 2), dim3(1024), 0, st, a, a.mid_list, a.mid_count,
+                a.mid2_list, a.mid2_count, a.mid2_list, a.mid2_count);
+            hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), dim3(mid_blocks), dim3(1024), 0,
+                               st, a, a.mid2_list, a.mid2_count, a.big_list, a.big_count);
+            if (a.scratch_slots > 0)
+                hipLaunchKernelGGL(map_strands_big, dim3(a.scratch_slots), dim3(256), 0, st, a);
+            return hipGetLastError();
         } else if (a.ix.cid_ok) {
             hipLaunchKernelGGL((map_strands_hash<512, kHashSlots, kHashCap, kMaxMapRead, 4>),
                                dim3(2 * mid_blocks), dim3(512), 0, st, a,

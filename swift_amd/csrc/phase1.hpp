@@ -25,6 +25,8 @@ constexpr int kHashSlots = 8192;       // cluster table of the hash tier (64 KB 
 constexpr int kHashCap = 6144;         // hits per strand the hash tier counts: table load
                                        // <= 0.75, so every insertion finds its slot
 constexpr int kMidCap = 16384;         // ... by one 1024-thread workgroup (128 KB LDS)
+constexpr int kLongFilterCap = 18432;  // short-read strands past kHashCap: the filter stage
+                                       // at 1024 threads (18 hits per thread)
 constexpr int kSmallBucket = 32;       // buckets sorted by one thread
 constexpr int kWaveBucket = 1024;      // buckets sorted by one wave in LDS
 constexpr int kScanChunk = 4096;       // hashes per scan workgroup
@@ -98,6 +100,8 @@ struct MapArgs {
     int32_t *mid_list, *mid_count;   // strands past the hash tier (kHashCap hits, or
                                      // kMidCap1 without 32-bit cluster ids)
     int32_t *big_list, *big_count;   // strands past kMidCap hits
+    int32_t *mid2_list, *mid2_count; // short reads: strands the long filter stage leaves to
+                                     // the workgroup sort
     int mid_blocks;                  // workgroups of map_strands_mid (0: none needed)
     int hash_first;                  // 1: every strand starts in the hash tier (no wave tier)
     int32_t *error;           // bit 0: a strand exceeded scratch_cap, bit 1: a read

@@ -319,6 +319,7 @@ static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     }
     if ((rc = ensure(w.big_list, 4 * ((size_t)ns + 16)))) return rc;
     if ((rc = ensure(w.mid_list, 4 * ((size_t)ns + 16)))) return rc;
+    if ((rc = ensure(w.mid2_list, 4 * ((size_t)ns + 16)))) return rc;
     m->ix = ix.dev();
     m->rd = rd;
     m->out = MapOut{w.ref.as<int8_t>(), w.shift.as<int32_t>(), w.pos.as<int32_t>(),
@@ -337,6 +338,8 @@ static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
     m->big_count = w.flags.as<int32_t>() + 2;
     m->mid_list = w.mid_list.as<int32_t>();
     m->mid_count = w.flags.as<int32_t>() + 3;
+    m->mid2_list = w.mid2_list.as<int32_t>();
+    m->mid2_count = w.flags.as<int32_t>() + 4;
     m->mid_blocks = bound > kMapCap ? 2 * 256 : 0;
     // Hash tier first when the index's mean bucket size predicts more hits
     // per strand than the wave tier holds: the wave tier would only count
@@ -348,8 +351,8 @@ static int prepare_map(const IndexObj &ix, const ReadsDev &rd, int max_read_len,
 }
 
 static int run_map(const MapArgs &m, hipStream_t st) {
-    // overflow count, error flags, big- and mid-strand counts
-    SWIFT_HIP_TRY(launch_fill(m.ovl_count, 4, 0, st));
+    // overflow count, error flags, big-, mid- and mid2-strand counts
+    SWIFT_HIP_TRY(launch_fill(m.ovl_count, 5, 0, st));
     SWIFT_HIP_TRY(launch_map(m, st, m.mid_blocks));
     return kOk;
 }

@@ -67,8 +67,8 @@ struct IndexObj {
 
 // Mapping / pipeline scratch (grows, never shrinks).
 struct MapWork {
-    DevArr ref, shift, pos, nbest, nbig, nhits, strand_seq, ovl_list, mid_list, big_list, flags,
-        scratch;
+    DevArr ref, shift, pos, nbest, nbig, nhits, strand_seq, ovl_list, mid_list, mid2_list,
+        big_list, flags, scratch;
     DevArr pair0, qoff, roff, coff, qlen, rlen, scan_sums;
 };
 

@@ -184,9 +184,11 @@ def filter_tier_reads():
     a true locus of about 25 hits among singleton clusters; (b) random reads
     on a k=8 index of 1 Mbp, a few hundred hits that are all clusters of one;
     (c) a 24-base motif at 100 positions on the k=12 tuple grid: 100
-    clusters tied at two hits, more than one wave lists; (d) 100-base reads
-    on a k=6 index of 1 Mbp, about 40 entries per bucket and 3,800 hits per
-    strand."""
+    clusters tied at two hits, more than one wave lists; (d) reads on a k=6
+    index of 1 Mbp, about 40 entries per bucket: 100 bases (3,800 hits per
+    strand), 250 bases (about 9,800 hits: the 1024-thread filter stage for
+    strands past 6,144) and 480 bases (about 19,000: past that stage's
+    18,432)."""
     rnd = random.Random(123)
     out = []
     g6 = bytes(rnd.choice(b"ACGT") for _ in range(300_000))
@@ -203,6 +205,10 @@ def filter_tier_reads():
     g6 = bytes(rnd.choice(b"ACGT") for _ in range(1_000_000))
     reads = [g6[p:p + 100] for p in (rnd.randrange(0, 999_000) for _ in range(30))]
     reads += [bytes(rnd.choice(b"ACGT") for _ in range(100)) for _ in range(5)]
+    # about 9,800 hits: past the first filter's 6,144, into the 1024-thread
+    # filter stage; about 19,000: past its 18,432, on to the sorting tiers
+    reads += [g6[p:p + 250] for p in (rnd.randrange(0, 999_000) for _ in range(12))]
+    reads += [g6[p:p + 480] for p in (rnd.randrange(0, 999_000) for _ in range(4))]
     out.append(([g6], 6, reads))
     return out
 
@@ -210,8 +216,9 @@ def filter_tier_reads():
 def test_map_filter_tier_paths(S, orc):
     """Hash-first mapping (mode 1) of short reads in the filter tier: the
     biggest clusters of two or more hits from the exact candidate table
-    (listed by one wave, or by table scans past 64 tied clusters), and
-    strands whose clusters are all single hits; every strand against the
+    (listed by one wave, or by table scans past 64 tied clusters), strands
+    whose clusters are all single hits, and strands past the first stage's
+    cap through the 1024-thread stage and beyond; every strand against the
     oracle, tie-set sizes and hit counts included."""
     before = S.get_map_mode()
     try:
