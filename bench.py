@@ -515,12 +515,13 @@ def c5_leg(index, g, off, lens, n_requests, seed=20261205, p1_per_kmer_read=None
 
 # the fixed arrival rate of the serving leg's headline latencies (reads/s)
 C5_RATE = 1_000_000
+C5_SLOTS = 4  # server slots of the serving leg (--c5-slots)
 
 
 def _c5_serve(index, packed, sizes, total_reads, n_requests, rng, res, seq_lens):
     import threading
     from swift_amd.serve import Server
-    with Server(index, nslots=4, mem_budget=48 << 30, max_batch_reads=8192) as srv:
+    with Server(index, nslots=C5_SLOTS, mem_budget=48 << 30, max_batch_reads=8192) as srv:
         for p in packed[:8]:  # warm the slots
             srv.result(srv.submit_packed(*p), strings=False)
         t0 = time.perf_counter()
@@ -575,7 +576,7 @@ def _c5_serve(index, packed, sizes, total_reads, n_requests, rng, res, seq_lens)
         st = srv.stats()
     res.update({"capacity_reads_per_s": round(capacity, 1), "capacity_s": round(cap_s, 4),
                 **fixed, "at_70pct_capacity": loaded,
-                "max_coalesced": st["max_coalesced"], "slots": 4})
+                "max_coalesced": st["max_coalesced"], "slots": C5_SLOTS})
     return res
 
 
@@ -745,6 +746,7 @@ def main():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the C4 hipGraph stream")
     ap.add_argument("--c5-requests", type=int, default=1500)
+    ap.add_argument("--c5-slots", type=int, default=4, help="server slots of the serving leg")
     ap.add_argument("--c4-only", action="store_true", help="only the genome-scale C4 leg "
                     "(and C5 on its index)")
     ap.add_argument("--c4-reads", type=int, default=10_000_000)
@@ -781,6 +783,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    global C5_SLOTS
+    C5_SLOTS = args.c5_slots
     if args.walk != "default" or args.waves >= 0:  # applies to every leg below
         from swift_amd import align as _align
         t = _align.get_tuning()
