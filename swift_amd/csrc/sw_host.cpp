@@ -303,8 +303,8 @@ static int max_tasks_for(int npairs) { return (npairs + kSlots32 - 1) / kSlots32
 
 static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
     int R = std::max(1, (maxq + kBatchG - 1) / kBatchG);
-    // coarse row classes round R up (bound for either mode)
-    R = std::min(kRBins, (R + kCoarseRows - 1) / kCoarseRows * kCoarseRows);
+    // coarse row classes round R up (bound for every mode)
+    R = std::min(kRBins, (R + kCoarseRowsSmall - 1) / kCoarseRowsSmall * kCoarseRowsSmall);
     int64_t steps = (maxr + kBatchG - 1 + 3) & ~3;
     int64_t w16 = (R + 7) / 8, w32 = (r32_class(R) + 15) / 16;
     int64_t t16 = (npairs + kSlots16 - 1) / kSlots16 + 2 * kRBins;
@@ -432,8 +432,13 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     // run one after another (or over the side streams) times a task; at most
     // 4 classes instead of 20.  Large batches keep every R (no padding rows).
     const int cmode = tuning().coarse_rows;
+    // the smallest batches with host-known sizes round to kCoarseRowsSmall:
+    // their class kernels run about one task each, one after another, so
+    // fewer classes is lower latency
     const int coarse =
-        cmode == 1 || (cmode < 0 && (w.nside == 0 || !sync_sizes || n < kCoarsePairs)) ? 1 : 0;
+        cmode == 1 || (cmode < 0 && (w.nside == 0 || !sync_sizes || n < kCoarsePairs))
+            ? (sync_sizes && n < kCoarseSmallPairs ? kCoarseRowsSmall : kCoarseRows)
+            : 0;
     auto pass_args = [&](int pass, const PairResult *prev, int f32, BinArgs &b, TaskArgs &t) {
         b = BinArgs{io.rseq, io.roff, io.qlen, io.rlen, prev, n, f32, w.bin_of.as<int32_t>(),
                     bins + (2 * pass) * kNumBins, cnt + kCntError, io.score, io.start, io.end,

@@ -115,6 +115,7 @@ struct Tuning {
 };
 // batches below this many pairs take the coarse row classes under coarse_rows -1
 constexpr int kCoarsePairs = 131072;
+constexpr int kCoarseSmallPairs = 16384;  // ... rounded to kCoarseRowsSmall below this
 Tuning &tuning();
 
 }  // namespace swift

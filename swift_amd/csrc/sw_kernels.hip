@@ -1073,7 +1073,7 @@ __device__ __forceinline__ int classify_pair(const BinArgs &a, int p) {
     int R = max(1, (ql + kBatchG - 1) / kBatchG);
     // coarse classes (small batches): fewer class kernels, each a pair's rows
     // spread over fewer lanes of its group; the rows past the query are padding
-    if (a.coarse) R = min(kRBins, (R + kCoarseRows - 1) / kCoarseRows * kCoarseRows);
+    if (a.coarse > 0) R = min(kRBins, (R + a.coarse - 1) / a.coarse * a.coarse);
     if (cls == 1) R = r32_class(R);
     int lb = min(kLenBins - 1, rl >> 3);
     // bin order: class, then R descending, then rlen descending

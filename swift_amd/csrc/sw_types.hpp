@@ -111,12 +111,14 @@ struct BinArgs {
     // empty pairs (query or reference of length 0) get their all-zero result
     // here and take no part in the kernels
     int32_t *score_out, *start_out, *end_out, *alen_out;
-    // 1: rows per lane rounded up to a multiple of kCoarseRows (at most
-    // kRBins / kCoarseRows classes; exact, the extra rows are padding)
+    // > 0: rows per lane rounded up to a multiple of `coarse` (kCoarseRows,
+    // or kCoarseRowsSmall for the smallest batches: at most kRBins / coarse
+    // classes; exact, the extra rows are padding); 0: every R its own class
     int coarse;
 };
 
 constexpr int kCoarseRows = 5;
+constexpr int kCoarseRowsSmall = 10;
 
 struct TaskArgs {
     const int32_t *bin_count;
