@@ -432,12 +432,15 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     // run one after another (or over the side streams) times a task; at most
     // 4 classes instead of 20.  Large batches keep every R (no padding rows).
     const int cmode = tuning().coarse_rows;
-    // the smallest batches with host-known sizes round to kCoarseRowsSmall:
-    // their class kernels run about one task each, one after another, so
-    // fewer classes is lower latency
+    // the smallest batches with host-known sizes round to kCoarseRowsSmall,
+    // or under kCoarseTinyPairs to a single class: their class kernels run
+    // about one task each, one after another, so fewer classes is lower
+    // latency
     const int coarse =
         cmode == 1 || (cmode < 0 && (w.nside == 0 || !sync_sizes || n < kCoarsePairs))
-            ? (sync_sizes && n < kCoarseSmallPairs ? kCoarseRowsSmall : kCoarseRows)
+            ? (!sync_sizes || n >= kCoarseSmallPairs ? kCoarseRows
+               : n >= kCoarseTinyPairs               ? kCoarseRowsSmall
+                                                     : kRBins)
             : 0;
     auto pass_args = [&](int pass, const PairResult *prev, int f32, BinArgs &b, TaskArgs &t) {
         b = BinArgs{io.rseq, io.roff, io.qlen, io.rlen, prev, n, f32, w.bin_of.as<int32_t>(),

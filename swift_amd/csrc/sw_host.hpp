@@ -116,6 +116,7 @@ struct Tuning {
 // batches below this many pairs take the coarse row classes under coarse_rows -1
 constexpr int kCoarsePairs = 131072;
 constexpr int kCoarseSmallPairs = 16384;  // ... rounded to kCoarseRowsSmall below this
+constexpr int kCoarseTinyPairs = 4096;    // ... and to one class (R = kRBins) below this
 Tuning &tuning();
 
 }  // namespace swift
