@@ -357,7 +357,7 @@ static int run_map(const MapArgs &m, hipStream_t st) {
     return kOk;
 }
 
-static int map_error(int flags) {
+int map_error(int flags) {
     if (flags & 2) {
         set_error("a read is longer than max_read_len");
         return kErrCapacity;

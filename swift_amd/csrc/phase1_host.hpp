@@ -89,6 +89,8 @@ int align_prepare(MapWork &w, const IndexObj &ix, const ReadsDev &rd, int max_re
 int align_enqueue(const MapArgs &m, const PairsDev &p, Workspace *sws, int nreads, int max_read_len,
                   const Scoring &sc, const swift_records &rec, hipStream_t st, bool sync);
 int map_check(const MapArgs &m, hipStream_t st);
+// the status of the mapping error flags word (map_check without the readback)
+int map_error(int flags);
 // device bytes align_reads_in may hold for nreads reads of <= max_read_len
 int64_t align_bytes_estimate(int nreads, int max_read_len, int max_hits);
 

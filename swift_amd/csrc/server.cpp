@@ -161,7 +161,12 @@ static int run_batch(Server &S, SrvSlot &sl, std::vector<SrvRequest *> &batch) {
     rec.cons_stride = cs;
     rec.capacity = (int32_t)cap;
     ReadsDev rd{sl.seq.as<char>(), sl.off.as<int64_t>(), sl.len.as<int32_t>(), nreads};
-    if ((rc = align_reads_in(sl.mw, sl.sws, *S.ix, rd, max_len, S.max_hits, S.sc, rec, sl.st)))
+    // the pipeline without its own final readback: the mapping error flags
+    // come back with the record count below
+    MapArgs m;
+    PairsDev p;
+    if ((rc = align_prepare(sl.mw, *S.ix, rd, max_len, S.max_hits, rec, &m, &p)) ||
+        (rc = align_enqueue(m, p, sl.sws, nreads, max_len, S.sc, rec, sl.st, true)))
         return rc;
     // consensus strings packed on the device, then one exact-size copy each
     int64_t *total_d = sl.nrec.as<int64_t>() + 1;
@@ -170,11 +175,14 @@ static int run_batch(Server &S, SrvSlot &sl, std::vector<SrvRequest *> &batch) {
                                    sl.pq.as<char>(), sl.pr.as<char>(), sl.st));
     if ((rc = sl.pin_out.ensure(64))) return rc;
     SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p, sl.nrec.ptr, 16, hipMemcpyDeviceToHost, sl.st));
+    SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p + 16, m.ovl_count, 8, hipMemcpyDeviceToHost, sl.st));
     SWIFT_HIP_TRY(hipStreamSynchronize(sl.st));
-    int32_t n;
+    int32_t n, mflags[2];
     int64_t packed;
     memcpy(&n, sl.pin_out.p, 4);
     memcpy(&packed, sl.pin_out.p + 8, 8);
+    memcpy(mflags, sl.pin_out.p + 16, 8);
+    if ((rc = map_error(mflags[1]))) return rc;
     if (n == 0) packed = 0;
     const size_t o_col = 0, o_rd = align8(kRecI32 * 4 * (size_t)n), o_q = o_rd + 8 * (size_t)n,
                  o_r = o_q + align8((size_t)packed), o_end = o_r + (size_t)packed;

@@ -371,8 +371,10 @@ constexpr int kCntError = 8;
 constexpr int kCntFlagged = 9;
 constexpr int kCntQueues = 16;
 constexpr int kQueuesPerPass = 2 * kRBins;
+// sync_sizes: the sizes of each pass are read back (synchronising); then
+// final_sync also waits for the pass's last kernels before returning
 static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStream_t st,
-                     bool sync_sizes);
+                     bool sync_sizes, bool final_sync = true);
 
 int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync_sizes) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -381,7 +383,9 @@ int sw_batch_run(const BatchIO &io, const Scoring &sc, hipStream_t st, bool sync
 
 int sw_batch_run_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st) {
     std::lock_guard<std::mutex> lk(w->mu);
-    return batch_run(*w, io, sc, st, true);
+    // the caller's later work is on the same stream, and it synchronises
+    // before it reads results
+    return batch_run(*w, io, sc, st, true, false);
 }
 
 int sw_batch_enqueue_in(Workspace *w, const BatchIO &io, const Scoring &sc, hipStream_t st) {
@@ -401,7 +405,7 @@ int sw_batch_check_in(Workspace *w, hipStream_t st) {
 }
 
 static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStream_t st,
-                     bool sync_sizes) {
+                     bool sync_sizes, bool final_sync) {
     if (io.npairs < 0) {
         set_error("negative pair count");
         return kErrInvalid;
@@ -542,6 +546,10 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     // (serving slots) one walk after the last class is cheaper: a walk kernel
     // is a latency-bound tail, so one instead of one per class.
     const bool class_walks = ba.walk_deferred && sync_sizes && w.nside > 0;
+    // With every class on one stream and host-known sizes, each pass's walk
+    // follows its last class directly, ahead of the next readback, so the
+    // GPU walks while the host waits for the flagged count.
+    const bool pass_walks = ba.walk_deferred && sync_sizes && !class_walks;
     auto launch_classes = [&](const TaskArgs &t, bool with16, int pass, const int32_t *seg_host,
                               const int64_t *steps_host) -> int {
         int live[2 * kRBins], nlive = 0;
@@ -606,6 +614,9 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     if ((rc = launch_classes(t1, !force32, 0, sync_sizes ? seg1 : nullptr,
                              sync_sizes ? steps1 : nullptr)))
         return rc;
+    if (pass_walks)  // pass 1's pairs (the flagged ones wait for pass 2)
+        SWIFT_HIP_TRY(launch_walk(ba, w.order.as<int32_t>(), nullptr, cnt + 1, n,
+                                  w.moves.as<uint32_t>(), kMoveWords, st));
 
     SWIFT_HIP_TRY(mark(2, st, w.timed));
     // ---- pass 2: int32 re-run of pairs flagged by the int16 kernel
@@ -650,22 +661,25 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             if ((rc = launch_classes(t2, false, 1, sync_sizes ? seg2 : nullptr,
                                      sync_sizes ? steps2 : nullptr)))
                 return rc;
+            if (pass_walks)  // the re-run pairs
+                SWIFT_HIP_TRY(launch_walk(ba, w.order2.as<int32_t>(), nullptr, cnt + 5, n,
+                                          w.moves.as<uint32_t>(), kMoveWords, st));
         }
     }
     w.stats.trace_words = words2;
 
     // ---- walk-back: fused into the forward kernels; or, deferred, after each
-    // class's sweep (host-known class sizes, side streams) or else here, over
-    // every binned pair of pass 1 (re-run pairs carry their pass-2 results by
-    // now)
+    // class's sweep (host-known class sizes, side streams) or each pass
+    // (host-known sizes, one stream), or else here, over every binned pair of
+    // pass 1 (re-run pairs carry their pass-2 results by now)
     SWIFT_HIP_TRY(mark(3, st, w.timed));
-    if (ba.walk_deferred && !class_walks) {
+    if (ba.walk_deferred && !class_walks && !pass_walks) {
         ba.order = w.order.as<int32_t>();
         SWIFT_HIP_TRY(launch_walk(ba, w.order.as<int32_t>(), nullptr, cnt + 1, n,
                                   w.moves.as<uint32_t>(), kMoveWords, st));
     }
     SWIFT_HIP_TRY(mark(4, st, w.timed));
-    if (sync_sizes) {
+    if (sync_sizes && final_sync) {
         SWIFT_HIP_TRY(hipStreamSynchronize(st));
         if (g_timing.on && w.timed) {
             for (int i = 0; i < 4; ++i)

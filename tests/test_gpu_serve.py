@@ -62,6 +62,39 @@ def test_results_same_alone_and_in_mix(world, orc):
         assert_records_match(g, orc.pipeline_mt(oix, r, 8), i)
 
 
+def test_served_int32_reruns_vs_oracle(orc):
+    """References holding bytes outside ACGTNacgtn (IUPAC R and Y, about 1 %
+    of the bases): their windows are re-run with byte semantics in the
+    int32 pass.  A serving slot runs its classes on one stream, where each
+    pass's walk follows the pass directly, so the re-run pairs take the
+    second pass's walk.  Every request's records against the oracle's."""
+    from swift_amd import search, synth
+    from swift_amd.serve import Server
+    rng = np.random.default_rng(57)
+    seqs = []
+    for n in (120_000, 80_000):
+        s = bytearray(synth.random_bases(rng, n, 0.41).tobytes())
+        for i in rng.choice(n, n // 100, replace=False):
+            s[i] = ord("R") if rng.random() < 0.5 else ord("Y")
+        seqs.append(bytes(s))
+    arrs = [np.frombuffer(x, np.uint8) for x in seqs]
+    reqs = []
+    for _ in range(8):
+        n = int(rng.integers(1, 60))
+        reads, _ = synth.sample_reads(rng, arrs, n, int(rng.integers(60, 201)), 0.01, 0.001)
+        reqs.append(reads)
+    ix = search.Index.from_sequences(seqs, 12, 1000)
+    oix = orc.Index(seqs, 12, 1000)
+    with Server(ix, nslots=2, mem_budget=4 << 30, max_batch_reads=1024) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        got = [srv.result(i)[0] for i in ids]
+    nrec = 0
+    for i, (g, r) in enumerate(zip(got, reqs)):
+        assert_records_match(g, orc.pipeline_mt(oix, r, 8), i)
+        nrec += g.n
+    assert nrec > 0
+
+
 def test_small_budget_many_batches(world):
     from swift_amd.serve import Server
     from swift_amd._lib import SwiftError
