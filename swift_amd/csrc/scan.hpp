@@ -111,11 +111,64 @@ scan_tile_apply(Load ld, Store st, const int32_t *n_dev, int n_host, const T *su
     }
 }
 
+// Short arrays (n_max <= kScanSmall): the whole scan in one 1024-thread
+// workgroup, 16 elements a thread, so one launch instead of three (small
+// serving batches are launch-bound).
+constexpr int kScanSmallPer = 16;
+constexpr int kScanSmall = 1024 * kScanSmallPer;
+
+template <class T, class Load, class Store>
+__global__ void __launch_bounds__(1024)
+scan_small(Load ld, Store st, const int32_t *n_dev, int n_host, const T *base0, T *total_a,
+           T *total_b) {
+    __shared__ T wsum[16];
+    const int n = min(scan_len(n_dev, n_host), n_host);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = (int)threadIdx.x * kScanSmallPer;
+    T v[kScanSmallPer], s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanSmallPer; ++j) {
+        v[j] = i0 + j < n ? ld(i0 + j) : T(0);
+        s += v[j];
+    }
+    T x = s;  // wave-inclusive scan of the thread sums, then the wave totals
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(x, d, 64);
+        if (lane >= d) x += o;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    T before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        T t = wsum[i];
+        before += i < w ? t : T(0);
+        all += t;
+    }
+    const T base = base0 ? *base0 : T(0);
+    T run = base + before + x - s;
+#pragma unroll
+    for (int j = 0; j < kScanSmallPer; ++j) {
+        if (i0 + j < n) st(i0 + j, run);
+        run += v[j];
+    }
+    if (threadIdx.x == 0) {
+        if (total_a) *total_a = base + all;
+        if (total_b) *total_b = base + all;
+    }
+}
+
 // n_max bounds the length (host-known); n_dev, when given, is the actual
 // length on the device.  sums needs n_max / kScanTile + 1 elements.
 template <class T, class Load, class Store>
 hipError_t launch_scan(Load ld, Store st, const int32_t *n_dev, int n_max, T *sums, const T *base0,
                        T *total_a, T *total_b, hipStream_t s) {
+    if (n_max <= kScanSmall) {
+        hipLaunchKernelGGL((scan_small<T, Load, Store>), dim3(1), dim3(1024), 0, s, ld, st, n_dev,
+                           n_max, base0, total_a, total_b);
+        return hipGetLastError();
+    }
     const int tiles = (n_max + kScanTile - 1) / kScanTile;
     if (tiles > 0)
         hipLaunchKernelGGL((scan_tile_sums<T, Load>), dim3(tiles), dim3(kScanThreads), 0, s, ld,

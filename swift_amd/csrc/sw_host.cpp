@@ -464,9 +464,22 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     // [2048 - pos, 2047] whose key is exact.
     uint32_t h_limit = (uint32_t)std::min(32767 - (4 * pos + 3) - 4, 4 * (2048 - pos));
 
-    // results start at "score 0": a pair no kernel reaches never walks
-    SWIFT_HIP_TRY(launch_fill(w.res.as<int32_t>(), n * (int)(sizeof(PairResult) / 4), 0, st));
-    SWIFT_HIP_TRY(launch_fill(cnt + kCntError, 2, 0, st));  // error flag, flagged pairs
+    // one launch zeroes the results (they start at "score 0": a pair no
+    // kernel reaches never walks), the error flag, the flagged count, the
+    // work-queue cursors and both passes' bin histograms
+    {
+        ZeroSpans z{};
+        z.p[0] = w.res.as<int32_t>();
+        z.n[0] = n * (int)(sizeof(PairResult) / 4);
+        z.p[1] = cnt + kCntError;
+        z.n[1] = kCntQueues + 2 * kQueuesPerPass - kCntError;
+        z.p[2] = bins;
+        z.n[2] = kNumBins;
+        z.p[3] = bins + 2 * kNumBins;
+        z.n[3] = kNumBins;
+        z.nspans = 4;
+        SWIFT_HIP_TRY(launch_zero(z, st));
+    }
     SWIFT_HIP_TRY(mark(0, st, w.timed));
     // ---- pass 1: bin every pair
     BinArgs b1;
@@ -524,7 +537,6 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
         w.stats.ntasks32 = seg1[2 * kRBins] - seg1[kRBins];
     }
     if (tuning().walk_deferred && (rc = w.moves.ensure((size_t)n * kMoveWords * 4))) return rc;
-    SWIFT_HIP_TRY(launch_fill(cnt + kCntQueues, 2 * kQueuesPerPass, 0, st));
     BatchArgs ba{io.qseq, io.qoff, io.qlen, io.rseq, io.roff, io.rlen, w.order.as<int32_t>(),
                  w.tasks.as<Task>(), nullptr, nullptr, nullptr, w.trace.as<uint32_t>(),
                  w.res.as<PairResult>(), w.ptr.as<PairTrace>(), kc16, h_limit,

@@ -24,6 +24,13 @@ int plain_trace_words(int R);
 int64_t plain_scratch_words(int qlen, int rlen);  // device words per pair
 void set_plain_chunk_pairs(int pairs);             // 0: chunks by the scratch budget alone
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st);
+// up to four int32 arrays zeroed by one kernel launch
+struct ZeroSpans {
+    int32_t *p[4];
+    int n[4];
+    int nspans;
+};
+hipError_t launch_zero(const ZeroSpans &z, hipStream_t st);
 
 // up to four device arrays of 32-bit words -> one host staging block
 struct Gather {

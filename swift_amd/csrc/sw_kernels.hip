@@ -1244,6 +1244,13 @@ __global__ void fill_i32(int32_t *p, int n, int32_t v) {
     if (i < n) p[i] = v;
 }
 
+// zeroes up to four arrays in one launch: blockIdx.y picks the array
+__global__ void __launch_bounds__(256) zero_spans(ZeroSpans z) {
+    int32_t *p = z.p[blockIdx.y];
+    const int n = z.n[blockIdx.y];
+    for (int i = blockIdx.x * 256 + (int)threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
+}
+
 // Copies a few small device arrays into one host staging block (mapped
 // pinned memory), so the host reads them all after a single stream
 // synchronisation instead of one copy each.
@@ -1411,8 +1418,6 @@ hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order
                           int64_t *task_words, const int64_t *base0, int64_t *total_words,
                           int64_t *scan_sums, int max_tasks, hipStream_t st) {
     int n = bin.npairs;
-    hipLaunchKernelGGL(fill_i32, dim3((kNumBins + 255) / 256), dim3(256), 0, st, bin.bin_count,
-                       kNumBins, 0);
     const int chunks = (n + kBinChunk - 1) / kBinChunk;
     if (n > 0) hipLaunchKernelGGL(sw_bin_classify, dim3(chunks), dim3(256), 0, st, bin);
     hipLaunchKernelGGL(sw_bin_scan, dim3(1), dim3(1024), 0, st, ta);
@@ -1426,6 +1431,15 @@ hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order
     }
     return launch_scan<int64_t>(LoadArr<int64_t>{task_words}, StoreTaskBase{tasks}, ta.ntasks,
                                 max_tasks, scan_sums, base0, total_words, nullptr, st);
+}
+
+hipError_t launch_zero(const ZeroSpans &z, hipStream_t st) {
+    int most = 0;
+    for (int k = 0; k < z.nspans; ++k) most = max(most, z.n[k]);
+    if (most <= 0) return hipSuccess;
+    const int blocks = min((most + 255) / 256, 2048);
+    hipLaunchKernelGGL(zero_spans, dim3(blocks, z.nspans), dim3(256), 0, st, z);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill(int32_t *p, int n, int32_t v, hipStream_t st) {

@@ -140,7 +140,8 @@ hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count,
 hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t *first,
                        const int32_t *count, int max_pairs, uint32_t *moves, int move_words,
                        hipStream_t st);
-// scan_sums: room for max_tasks / 2048 + 1 int64 values
+// scan_sums: room for max_tasks / 2048 + 1 int64 values; bin.bin_count
+// must be zero on entry
 hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order, Task *tasks,
                           int64_t *task_words, const int64_t *base0, int64_t *total_words,
                           int64_t *scan_sums, int max_tasks, hipStream_t st);
