@@ -164,11 +164,14 @@ int swift_set_tuning(int waves_per_cu, int allow_int16, int walk_deferred);
 int swift_get_tuning(int *waves_per_cu, int *allow_int16, int *walk_deferred);
 
 /* Row classes of a batched call: each pair runs in the kernel of its rows
- * per lane R = ceil(query / 16).  1 = R rounded up to a multiple of 5 (at
- * most 4 class kernels: lower latency for small batches, some padding rows);
- * 0 = every R its own class; -1 (default) = 1 for batches under 131072 pairs,
- * inside graph capture and on serving slots, else 0.  No counterpart in the
- * reference; results are identical in every mode. */
+ * per lane R = ceil(query / 16).  1 = coarse classes: R rounded up to a
+ * multiple of 5 (at most 4 class kernels: lower latency for small batches,
+ * some padding rows); when the call reads its sizes back (not under graph
+ * capture), to a multiple of 10 under 16384 pairs, and under 4096 pairs one
+ * class in the wide form (one 64-lane group per wave, 5 rows per lane, two
+ * pairs per task).  0 = every R its own class; -1 (default) = 1 for batches
+ * under 131072 pairs, inside graph capture and on serving slots, else 0.
+ * No counterpart in the reference; results are identical in every mode. */
 int swift_set_coarse_rows(int mode);
 
 /* Pairs per chunk of a swift_align_sequences call (0, the default: as many

@@ -221,8 +221,10 @@ def get_tuning() -> dict:
 
 
 def set_coarse_rows(mode: int) -> None:
-    """1: rows per lane rounded up to a multiple of 5 (at most 4 row-class
-    kernels); 0: every R its own class; -1: chosen per call by batch size."""
+    """1: coarse row classes (rows per lane rounded up to a multiple of 5;
+    with the call's sizes read back, of 10 under 16,384 pairs, and under
+    4,096 pairs one class in the wide form, a 64-lane group per wave);
+    0: every R its own class; -1: chosen per call by batch size."""
     check(lib().swift_set_coarse_rows(int(mode)))
 
 

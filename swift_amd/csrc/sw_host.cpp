@@ -299,7 +299,13 @@ void set_plain_chunk_pairs(int pairs) { g_plain_chunk_pairs = std::max(pairs, 0)
 // 2-bit walk moves per pair id: room for the longest path any binned pair can
 // have (the kernel limits, not the caller's maxq/maxr, bound what is binned)
 constexpr int kMoveWords = (kMaxBatchQ + kMaxBatchR) / 16 + 1;
-static int max_tasks_for(int npairs) { return (npairs + kSlots32 - 1) / kSlots32 + 2 * kRBins + 1; }
+// tasks of one pass: the int32 class's 4 pairs per task, or the wide form's
+// 2 (batches under kCoarseTinyPairs)
+static int max_tasks_for(int npairs) {
+    return std::max((npairs + kSlots32 - 1) / kSlots32,
+                    (std::min(npairs, kCoarseTinyPairs) + kSlotsWide - 1) / kSlotsWide) +
+           2 * kRBins + 1;
+}
 
 static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
     int R = std::max(1, (maxq + kBatchG - 1) / kBatchG);
@@ -309,7 +315,16 @@ static int64_t worst_trace_words(int npairs, int maxq, int maxr) {
     int64_t w16 = (R + 7) / 8, w32 = (r32_class(R) + 15) / 16;
     int64_t t16 = (npairs + kSlots16 - 1) / kSlots16 + 2 * kRBins;
     int64_t t32 = (npairs + kSlots32 - 1) / kSlots32 + 2 * kRBins;
-    return (t16 * w16 + 2 * t32 * w32) * steps * 64;
+    int64_t words = (t16 * w16 + 2 * t32 * w32) * steps * 64;
+    if (npairs < kCoarseTinyPairs) {
+        // one class (R = kRBins): the wide form, one trace word per step
+        // and lane, 2 pairs per task, rlen + 63 steps; int32 tasks R = 20
+        int64_t wsteps = (maxr + kWideG - 1 + 3) & ~3;
+        int64_t tw = (npairs + kSlotsWide - 1) / kSlotsWide;
+        int64_t w32t = (r32_class(kRBins) + 15) / 16;
+        words = std::max(words, tw * wsteps * 64 + 2 * t32 * w32t * steps * 64);
+    }
+    return words;
 }
 
 // totals layout (int64): [pass] trace words, then from kTotSteps the sweep
@@ -446,6 +461,9 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
                : n >= kCoarseTinyPairs               ? kCoarseRowsSmall
                                                      : kRBins)
             : 0;
+    // a single class (the smallest batches) runs the wide form: its few
+    // tasks are each one wave, and a wave's sweep is the batch's latency
+    const int wide = coarse == kRBins ? 1 : 0;
     auto pass_args = [&](int pass, const PairResult *prev, int f32, BinArgs &b, TaskArgs &t) {
         b = BinArgs{io.rseq, io.roff, io.qlen, io.rlen, prev, n, f32, w.bin_of.as<int32_t>(),
                     bins + (2 * pass) * kNumBins, cnt + kCntError, io.score, io.start, io.end,
@@ -453,7 +471,8 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
         t = TaskArgs{bins + (2 * pass) * kNumBins, bins + (2 * pass + 1) * kNumBins,
                      segs + (3 * pass) * NS, segs + (3 * pass + 1) * NS,
                      segs + (3 * pass + 2) * NS, cnt + 4 * pass, cnt + 4 * pass + 1,
-                     reinterpret_cast<unsigned long long *>(tot + kTotSteps + pass * 2 * kRBins)};
+                     reinterpret_cast<unsigned long long *>(tot + kTotSteps + pass * 2 * kRBins),
+                     wide};
     };
     KeyConsts kc16 = make_consts(sc, kOneHot), kc32 = make_consts(sc, kBytes);
     int pos = std::max(std::max(std::max(sc.match, sc.mismatch), std::max(sc.gap_open, sc.gap_ext)), 0);
@@ -608,9 +627,13 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             b.kc = cls == 0 ? kc16 : kc32;
             hipStream_t s2 = nside ? w.side[lane_of[seg]] : st;
             const int wpc = w.waves_per_cu >= 0 ? w.waves_per_cu : tuning().waves_per_cu;
-            SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, wpc, s2));
+            const bool wseg = t.wide && seg == 0;
+            if (wseg)
+                SWIFT_HIP_TRY(launch_batch_wide(b, w.cu_count, s2));
+            else
+                SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, wpc, s2));
             if (class_walks && seg_host) {  // per-class walk
-                const int slots = cls == 0 ? kSlots16 : kSlots32;
+                const int slots = wseg ? kSlotsWide : cls == 0 ? kSlots16 : kSlots32;
                 const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;
                 SWIFT_HIP_TRY(launch_walk(b, b.order, t.seg_first + seg, t.seg_len + seg, maxp,
                                           w.moves.as<uint32_t>(), kMoveWords, s2));

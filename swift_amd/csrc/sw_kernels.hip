@@ -583,19 +583,25 @@ sw_plain_kernel(const char *__restrict__ qseq, const char *__restrict__ rseq, in
 // ---------------------------------------------------------------------------
 // Batched persistent kernels.
 // ---------------------------------------------------------------------------
-constexpr int kPad = kBatchG;                        // zero columns before column 0
-constexpr int kGroupLds = kPad + kBatchCols + kBatchG; // LDS words per lane group
+// LDS words per lane group of G lanes: G zero columns before column 0, the
+// reference columns, and G columns of tail
+template <int G>
+constexpr int group_lds_words() { return G + kBatchCols + G; }
 
-// One wave task: 4 groups of 16 lanes; PK -> 2 pairs per group (halves).
-template <int R, bool PK, int POL>
+// One wave task: 64 / G groups of G lanes (G = 16: 4 groups; G = 64, the
+// wide form: one); PK -> 2 pairs per group (halves).
+template <int G, int R, bool PK, int POL>
 __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uint32_t *lds) {
-    using S = Sweep<kBatchG, R, PK, POL>;
+    using S = Sweep<G, R, PK, POL>;
     constexpr int PPG = PK ? 2 : 1;  // pairs per group
+    constexpr int NG = 64 / G;       // groups per wave
+    constexpr int PAD = G;           // zero columns before column 0
+    constexpr int GL = group_lds_words<G>();
     const int lane = threadIdx.x & 63;
-    const int grp = lane >> 4, g = lane & 15;
-    // reference codes at [kPad + j]; the kPad leading entries and the tail
+    const int grp = lane / G, g = lane % G;
+    // reference codes at [PAD + j]; the PAD leading entries and the tail
     // are read by lanes outside the pair's columns (their results unused)
-    uint32_t *rr_lds = lds + grp * kGroupLds;
+    uint32_t *rr_lds = lds + grp * GL;
 
     // the group's pairs
     int pid[PPG], ql[PPG], rl[PPG];
@@ -610,11 +616,11 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
         ro[h] = pid[h] >= 0 ? a.roff[pid[h]] : 0;
     }
     // reference codes of the group's pairs into LDS (both halves packed),
-    // padding up to the last column any lane reaches (ncols + kBatchG - 2)
+    // padding up to the last column any lane reaches (ncols + G - 2)
     // and the step-ahead prefetch of it
-    const int ncols = tk.steps - (kBatchG - 1);
+    const int ncols = tk.steps - (G - 1);
     uint32_t exotic = 0;  // bit h: the pair's reference holds a byte outside ACGTNacgtn
-    for (int j = g; j < ncols + kBatchG; j += kBatchG) {
+    for (int j = g; j < ncols + G; j += G) {
         uint32_t v = 0;
 #pragma unroll
         for (int h = 0; h < PPG; ++h) {
@@ -627,13 +633,13 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
             if constexpr (PK) code = column_selector(code, h);
             v |= code << (16 * h);
         }
-        rr_lds[kPad + j] = v;
+        rr_lds[PAD + j] = v;
     }
     if constexpr (PK) {
 #pragma unroll
-        for (int s2 = kBatchG / 2; s2 >= 1; s2 >>= 1) exotic |= __shfl_xor(exotic, s2, kBatchG);
+        for (int s2 = G / 2; s2 >= 1; s2 >>= 1) exotic |= __shfl_xor(exotic, s2, G);
     }
-    if (g < kPad)
+    if (g < PAD)
         rr_lds[g] = PK ? column_selector(kRefPad, 0) | column_selector(kRefPad, 1) << 16 : 0u;
     S sw;
     sw.init(a.kc);
@@ -657,13 +663,13 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     __builtin_amdgcn_wave_barrier();
 
     uint32_t *tr = a.trace + tk.trace_base;
-    uint32_t rr_next = rr_lds[kPad - g];  // column of step 0
+    uint32_t rr_next = rr_lds[PAD - g];  // column of step 0
     auto run = [&](auto masked, int t) {
         int j = t - g;
         uint32_t rr = rr_next;
         // prefetch the next step's column (clamped: steps past the last
         // granule boundary read a stale word whose results are unused)
-        rr_next = rr_lds[min(kPad + 1 + j, kGroupLds - 1)];
+        rr_next = rr_lds[min(PAD + 1 + j, GL - 1)];
         uint32_t vmask = 0;
         if constexpr (decltype(masked)::value) {
 #pragma unroll
@@ -686,7 +692,7 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
             if ((t & 15) == 15 || t == T - 1) sw.block_end(splat<true>((t >> 4) << 5));
         }
     } else {
-        const int lo = min(kBatchG - 1, T), hi = max(lo, tk.min_rlen);
+        const int lo = min(G - 1, T), hi = max(lo, tk.min_rlen);
         sweep_steps(T, lo, hi, run);
     }
 
@@ -714,7 +720,7 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
                 bcol = cv;
             }
         }
-        gres[h] = group_best<kBatchG>(bh, brow, bcol, R, PK, a.h_limit, (exotic >> h) & 1);
+        gres[h] = group_best<G>(bh, brow, bcol, R, PK, a.h_limit, (exotic >> h) & 1);
         if (g == 0 && pid[h] >= 0) {
             a.res[pid[h]] = PairResult{gres[h].score, gres[h].row, gres[h].col, gres[h].flag};
             if (gres[h].flag) atomicAdd(a.nflagged, 1);
@@ -722,11 +728,11 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     }
 
     const int s = lane;
-    const int sg = min(s / PPG, 3), sh = s % PPG;
+    const int sg = min(s / PPG, NG - 1), sh = s % PPG;
     if (a.walk_deferred) {
         // record where the pair's trace lives; sw_walk_kernel walks it
         if (s < tk.npairs)
-            a.ptr[a.order[tk.first + s]] = PairTrace{tk.trace_base, sg * kBatchG, (int16_t)R,
+            a.ptr[a.order[tk.first + s]] = PairTrace{tk.trace_base, sg * G, (int16_t)R,
                                                      (int8_t)S::W, (int8_t)(sh | (PK ? 2 : 0))};
         __builtin_amdgcn_wave_barrier();
         return;
@@ -738,10 +744,10 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     int w_score = 0, w_row = 0, w_col = 0, w_flag = 1;
 #pragma unroll
     for (int h = 0; h < PPG; ++h) {
-        int v0 = __shfl(gres[h].score, sg * kBatchG, 64);
-        int v1 = __shfl(gres[h].row, sg * kBatchG, 64);
-        int v2 = __shfl(gres[h].col, sg * kBatchG, 64);
-        int v3 = __shfl(gres[h].flag, sg * kBatchG, 64);
+        int v0 = __shfl(gres[h].score, sg * G, 64);
+        int v1 = __shfl(gres[h].row, sg * G, 64);
+        int v2 = __shfl(gres[h].col, sg * G, 64);
+        int v3 = __shfl(gres[h].flag, sg * G, 64);
         if (h == sh) {
             w_score = v0;
             w_row = v1;
@@ -751,7 +757,7 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     }
     if (s < tk.npairs && !w_flag) {
         const int p = a.order[tk.first + s];
-        const PairTrace pt{tk.trace_base, sg * kBatchG, (int16_t)R, (uint8_t)S::W,
+        const PairTrace pt{tk.trace_base, sg * G, (int16_t)R, (uint8_t)S::W,
                            (int8_t)(sh | (PK ? 2 : 0))};
         const int64_t co = a.coff[p];
         WalkOut wo = walk_pair(a.trace, w_score, w_row, w_col, pt, a.qseq + a.qoff[p],
@@ -1018,7 +1024,7 @@ template <int R, bool PK, int POL>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu(R, PK))))
 sw_batch_kernel(BatchArgs a) {
-    __shared__ uint32_t lds[4][4 * kGroupLds];
+    __shared__ uint32_t lds[4][4 * group_lds_words<kBatchG>()];
     const int wave = threadIdx.x >> 6;
     const int lo = *a.task_lo, hi = *a.task_hi;
     if (lo >= hi) return;
@@ -1028,7 +1034,28 @@ sw_batch_kernel(BatchArgs a) {
         tid = __builtin_amdgcn_readfirstlane(__shfl(tid, 0, 64)) + lo;
         if (tid >= hi) break;
         const Task tk = a.tasks[tid];
-        run_task<R, PK, POL>(a, tk, lds[wave]);
+        run_task<kBatchG, R, PK, POL>(a, tk, lds[wave]);
+    }
+}
+
+// The wide form for batches too small to fill the chip: one 64-lane group
+// per wave (the lane shift a wave_shr), kWideRows rows per lane, two packed
+// pairs per task.  A 16-lane group sweeps R = 20 rows per lane over rlen + 15
+// steps; a 64-lane group 5 rows over rlen + 63, so one task's latency, which
+// is all a small batch waits for, drops about 3.5x for the same total work.
+template <int R>
+__global__ void __launch_bounds__(256) sw_batch_wide_kernel(BatchArgs a) {
+    __shared__ uint32_t lds[4][group_lds_words<kWideG>()];
+    const int wave = threadIdx.x >> 6;
+    const int lo = *a.task_lo, hi = *a.task_hi;
+    if (lo >= hi) return;
+    for (;;) {
+        int tid = 0;
+        if ((threadIdx.x & 63) == 0) tid = atomicAdd(a.next_task, 1);
+        tid = __builtin_amdgcn_readfirstlane(__shfl(tid, 0, 64)) + lo;
+        if (tid >= hi) break;
+        const Task tk = a.tasks[tid];
+        run_task<kWideG, R, true, kOneHot>(a, tk, lds[wave]);
     }
 }
 
@@ -1140,7 +1167,7 @@ __global__ void __launch_bounds__(1024) sw_bin_scan(TaskArgs a) {
         const int seg = tid, cls = seg / kRBins;
         int32_t len = 0;
         for (int lb = 0; lb < kLenBins; ++lb) len += a.bin_count[seg * kLenBins + lb];
-        const int slots = cls == 0 ? kSlots16 : kSlots32;
+        const int slots = cls == 0 ? (a.wide && seg == 0 ? kSlotsWide : kSlots16) : kSlots32;
         a.seg_first[seg] = a.bin_cursor[seg * kLenBins];
         a.seg_len[seg] = len;
         seg_tasks[seg] = (len + slots - 1) / slots;
@@ -1187,12 +1214,18 @@ __device__ __forceinline__ void build_task(int t, const int32_t *seg_task0,
                                            const int32_t *seg_first, const int32_t *seg_len,
                                            const int32_t *order, const int32_t *rlen,
                                            Task *tasks, int64_t *task_words,
-                                           unsigned long long *steps_acc) {
+                                           unsigned long long *steps_acc, int wide) {
     int seg = 0;
     while (seg + 1 < 2 * kRBins && seg_task0[seg + 1] <= t) ++seg;
     int cls = seg / kRBins;
     int R = kRBins - (seg % kRBins);
-    int slots = cls == 0 ? 8 : 4;
+    int slots = cls == 0 ? kSlots16 : kSlots32;
+    int G = kBatchG;
+    if (wide && seg == 0) {
+        slots = kSlotsWide;
+        G = kWideG;
+        R = kWideRows;
+    }
     int k = t - seg_task0[seg];
     int first = seg_first[seg] + k * slots;
     int n = min(slots, seg_len[seg] - k * slots);
@@ -1207,7 +1240,7 @@ __device__ __forceinline__ void build_task(int t, const int32_t *seg_task0,
     tk.first = first;
     tk.npairs = (int16_t)n;
     tk.rows = (int16_t)R;
-    tk.steps = mx + kBatchG - 1;
+    tk.steps = mx + G - 1;
     tk.min_rlen = mn;
     tasks[t] = tk;
     int kpw = cls == 0 ? 8 : 16;
@@ -1219,7 +1252,7 @@ __device__ __forceinline__ void build_task(int t, const int32_t *seg_task0,
 __global__ void __launch_bounds__(256)
 sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_first, const int32_t *seg_len,
                const int32_t *ntasks_p, const int32_t *order, const int32_t *rlen, Task *tasks,
-               int64_t *task_words, unsigned long long *seg_steps) {
+               int64_t *task_words, unsigned long long *seg_steps, int wide) {
     // per-segment sums of the tasks' sweep steps (the host's cost estimate
     // of each class kernel), reduced in LDS before one atomic per segment
     __shared__ unsigned long long steps_acc[2 * kRBins];
@@ -1228,7 +1261,7 @@ sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_first, const int32_t
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int ntasks = *ntasks_p;
     if (t < ntasks) build_task(t, seg_task0, seg_first, seg_len, order, rlen, tasks, task_words,
-                               steps_acc);
+                               steps_acc, wide);
     __syncthreads();
     if (threadIdx.x < 2 * kRBins && steps_acc[threadIdx.x])
         atomicAdd(&seg_steps[threadIdx.x], steps_acc[threadIdx.x]);
@@ -1373,6 +1406,19 @@ static int full_grid(K kernel, int cu_count, int waves_per_cu) {
     return b * cu_count;
 }
 
+hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, hipStream_t st) {
+    auto k = sw_batch_wide_kernel<kWideRows>;
+    static int blocks = -1;  // per CU, at full occupancy
+    if (blocks < 0) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, 256, 0) != hipSuccess || b < 1)
+            b = 2;
+        blocks = b;
+    }
+    hipLaunchKernelGGL(k, dim3(blocks * cu_count), dim3(256), 0, st, ba);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
                         hipStream_t st) {
     if (int16) {
@@ -1427,7 +1473,7 @@ hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order
     if (max_tasks > 0) {
         hipLaunchKernelGGL(sw_build_tasks, dim3((max_tasks + 255) / 256), dim3(256), 0, st,
                            ta.seg_task0, ta.seg_first, ta.seg_len, ta.ntasks, order, bin.rlen,
-                           tasks, task_words, ta.seg_steps);
+                           tasks, task_words, ta.seg_steps, ta.wide);
     }
     return launch_scan<int64_t>(LoadArr<int64_t>{task_words}, StoreTaskBase{tasks}, ta.ntasks,
                                 max_tasks, scan_sums, base0, total_words, nullptr, st);

@@ -51,6 +51,10 @@ constexpr int kMaxBatchQ = kBatchG * kRBins;   // 320
 constexpr int kMaxBatchR = kBatchCols;         // 480
 constexpr int kSlots16 = 8;                    // pairs per wave task, int16 packed
 constexpr int kSlots32 = 4;                    // pairs per wave task, int32
+// wide form (small batches): one 64-lane group per wave, packed int16
+constexpr int kWideG = 64;
+constexpr int kWideRows = kMaxBatchQ / kWideG;  // 5
+constexpr int kSlotsWide = 2;
 
 __host__ __device__ inline int r32_class(int r) { return r <= 2 ? 2 : (r + 1) & ~1; }
 
@@ -129,11 +133,16 @@ struct TaskArgs {
     int32_t *ntasks;
     int32_t *npairs_binned;
     unsigned long long *seg_steps;  // sweep steps summed over each segment's tasks
+    // 1: segment 0 (int16, R = kRBins) runs the wide form: kSlotsWide pairs
+    // per task, kWideRows rows per lane, rlen + kWideG - 1 steps
+    int wide;
 };
 
 // launch helpers (sw_kernels.hip)
 hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
                         hipStream_t st);
+// the wide form's kernel (int16, kWideRows rows per lane)
+hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, hipStream_t st);
 // deferred walk-back over pairs order[*first .. *first + *count) (first may
 // be null: 0; max_pairs bounds *count); moves holds move_words words per pair
 // id, move_words >= (kMaxBatchQ + kMaxBatchR) / 16 + 1

@@ -119,15 +119,54 @@ def test_plain_exit_on_bad_score(A):
         A.align_sequences([b"ACGT"], [b"ACGT"], scoring=A.Scoring(2.5, -1, -10, -1))
 
 
-def test_batch_uneven_vs_oracle(A, orc):
-    """Batched path over every row class: query 1..320, reference 1..480."""
+@pytest.mark.parametrize("coarse", [0, -1])
+def test_batch_uneven_vs_oracle(A, orc, coarse):
+    """Batched path, query 1..320, reference 1..480: every 16-lane row class
+    (coarse 0), and the wide form that a batch this small takes by default
+    (one 64-lane group per wave, two pairs per task)."""
     rnd = random.Random(11)
     qs, rs = rand_pairs(rnd, 1500, 320, 480, [b"ACGT", b"ACGTN", b"AC", b"ACGTacgtn"])
-    res = A.sw_batch(qs, rs)
-    st = A.batch_stats()
-    assert st["ntasks16"] > 0
+    before = A.get_coarse_rows()
+    try:
+        A.set_coarse_rows(coarse)
+        res = A.sw_batch(qs, rs)
+        st = A.batch_stats()
+    finally:
+        A.set_coarse_rows(before)
+    if coarse == 0:
+        assert 0 < st["ntasks16"] <= 1500 // 8 + 20
+    else:
+        assert st["ntasks16"] == 750  # the wide form: 2 pairs per task
     for i in range(len(qs)):
         compare(res.record(i), orc.sw(rs[i], qs[i]), i)
+
+
+def test_batch_wide_form_limits(A, orc):
+    """The wide form at its limits: 4,095 pairs (the largest batch that takes
+    it) with the longest queries and references (320 x 480, 543 steps), and
+    packed halves of very different lengths."""
+    rnd = random.Random(29)
+    qs, rs = [], []
+    for i in range(4095):
+        if i % 3 == 0:
+            ql, rl = 320, 480
+        elif i % 3 == 1:
+            ql, rl = rnd.randint(1, 40), rnd.randint(1, 60)
+        else:
+            ql, rl = rnd.randint(200, 320), rnd.randint(300, 480)
+        reg = bytes(rnd.choice(b"ACGT") for _ in range(rl))
+        s0 = rnd.randint(0, max(0, rl - ql))
+        q = bytearray(reg[s0:s0 + ql].ljust(ql, b"A"))
+        for _ in range(ql // 30):
+            q[rnd.randrange(ql)] = rnd.choice(b"ACGT")
+        qs.append(bytes(q))
+        rs.append(reg)
+    res = A.sw_batch(qs, rs)
+    assert A.batch_stats()["ntasks16"] == 2048
+    sc, st, en, al, qc, rc = orc.sw_batch(qs, rs, nthreads=8)
+    assert np.array_equal(res.score, sc) and np.array_equal(res.start, st)
+    assert np.array_equal(res.end, en) and np.array_equal(res.length, al)
+    assert res.qry_cons == qc and res.ref_cons == rc
 
 
 def test_batch_c3_shapes_vs_oracle(A, orc):
