@@ -628,10 +628,11 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
             hipStream_t s2 = nside ? w.side[lane_of[seg]] : st;
             const int wpc = w.waves_per_cu >= 0 ? w.waves_per_cu : tuning().waves_per_cu;
             const bool wseg = t.wide && seg == 0;
+            const int ntk = seg_host ? seg_host[seg + 1] - seg_host[seg] : 0;
             if (wseg)
-                SWIFT_HIP_TRY(launch_batch_wide(b, w.cu_count, s2));
+                SWIFT_HIP_TRY(launch_batch_wide(b, w.cu_count, ntk, s2));
             else
-                SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, wpc, s2));
+                SWIFT_HIP_TRY(launch_batch(b, cls == 0, R, w.cu_count, wpc, ntk, s2));
             if (class_walks && seg_host) {  // per-class walk
                 const int slots = wseg ? kSlotsWide : cls == 0 ? kSlots16 : kSlots32;
                 const int maxp = (seg_host[seg + 1] - seg_host[seg]) * slots;

@@ -603,37 +603,85 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     // are read by lanes outside the pair's columns (their results unused)
     uint32_t *rr_lds = lds + grp * GL;
 
-    // the group's pairs
+    // the group's pairs.  The loads are unconditional (a slot past the
+    // task's pairs reads its first pair and is then cleared; byte loads are
+    // clamped into the sequence, whose first byte exists for every binned
+    // pair) so that each level of the order -> lengths -> bytes chain issues
+    // as one batch: one memory round trip per level instead of one per load.
     int pid[PPG], ql[PPG], rl[PPG];
     int64_t qo[PPG], ro[PPG];
 #pragma unroll
     for (int h = 0; h < PPG; ++h) {
-        int slot = grp * PPG + h;
-        pid[h] = slot < tk.npairs ? a.order[tk.first + slot] : -1;
-        ql[h] = pid[h] >= 0 ? a.qlen[pid[h]] : 0;
-        rl[h] = pid[h] >= 0 ? a.rlen[pid[h]] : 0;
-        qo[h] = pid[h] >= 0 ? a.qoff[pid[h]] : 0;
-        ro[h] = pid[h] >= 0 ? a.roff[pid[h]] : 0;
+        const int slot = grp * PPG + h;
+        pid[h] = a.order[tk.first + (slot < tk.npairs ? slot : 0)];
+    }
+#pragma unroll
+    for (int h = 0; h < PPG; ++h) {
+        const int p = pid[h];
+        ql[h] = a.qlen[p];
+        rl[h] = a.rlen[p];
+        qo[h] = a.qoff[p];
+        ro[h] = a.roff[p];
+    }
+#pragma unroll
+    for (int h = 0; h < PPG; ++h) {
+        if (grp * PPG + h >= tk.npairs) {
+            pid[h] = -1;
+            ql[h] = rl[h] = 0;
+            qo[h] = ro[h] = 0;
+        }
+    }
+    // the query rows: raw bytes now (in flight during the reference loads
+    // below), converted to match tables afterwards
+    S sw;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = g * R + r;
+        uint32_t v[PPG];
+#pragma unroll
+        for (int h = 0; h < PPG; ++h) {
+            // unconditional load of an in-range byte, then the select: a
+            // load under a branch is waited for before the branch joins
+            const uint32_t c = (unsigned char)a.qseq[qo[h] + max(0, min(row, ql[h] - 1))];
+            v[h] = row < ql[h] ? c : 256u;
+        }
+        sw.q[r] = v[0];
+        if constexpr (PK) sw.q2[r] = v[PPG - 1];
     }
     // reference codes of the group's pairs into LDS (both halves packed),
     // padding up to the last column any lane reaches (ncols + G - 2)
-    // and the step-ahead prefetch of it
+    // and the step-ahead prefetch of it.  Loaded in batches of NB columns
+    // per lane, each batch's loads issued together.
     const int ncols = tk.steps - (G - 1);
+    constexpr int NB = 8;
     uint32_t exotic = 0;  // bit h: the pair's reference holds a byte outside ACGTNacgtn
-    for (int j = g; j < ncols + G; j += G) {
-        uint32_t v = 0;
+    for (int j0 = g; j0 < ncols + G; j0 += NB * G) {
+        unsigned char cb[NB][PPG];
 #pragma unroll
-        for (int h = 0; h < PPG; ++h) {
-            uint32_t code = PK ? kRefPad : 0u;
-            if (j < rl[h]) {
-                unsigned char c = (unsigned char)a.rseq[ro[h] + j];
-                code = PK ? ref_index_bf(c) : byte_rcode(c);
-                if (PK && code == kRefOther) exotic |= 1u << h;
-            }
-            if constexpr (PK) code = column_selector(code, h);
-            v |= code << (16 * h);
+        for (int i = 0; i < NB; ++i) {
+            const int j = j0 + i * G;
+#pragma unroll
+            for (int h = 0; h < PPG; ++h)
+                cb[i][h] = (unsigned char)a.rseq[ro[h] + max(0, min(j, rl[h] - 1))];
         }
-        rr_lds[PAD + j] = v;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int j = j0 + i * G;
+            if (j < ncols + G) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int h = 0; h < PPG; ++h) {
+                    uint32_t code = PK ? kRefPad : 0u;
+                    if (j < rl[h]) {
+                        code = PK ? ref_index_bf(cb[i][h]) : byte_rcode(cb[i][h]);
+                        if (PK && code == kRefOther) exotic |= 1u << h;
+                    }
+                    if constexpr (PK) code = column_selector(code, h);
+                    v |= code << (16 * h);
+                }
+                rr_lds[PAD + j] = v;
+            }
+        }
     }
     if constexpr (PK) {
 #pragma unroll
@@ -641,22 +689,16 @@ __device__ __forceinline__ void run_task(const BatchArgs &a, const Task &tk, uin
     }
     if (g < PAD)
         rr_lds[g] = PK ? column_selector(kRefPad, 0) | column_selector(kRefPad, 1) << 16 : 0u;
-    S sw;
     sw.init(a.kc);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        int row = g * R + r;
-        uint32_t v[PPG];
-#pragma unroll
-        for (int h = 0; h < PPG; ++h) {
-            v[h] = 0;
-            if (row < ql[h]) {
-                unsigned char c = (unsigned char)a.qseq[qo[h] + row];
-                v[h] = PK ? query_table(query_mask_bf(c)) : byte_qcode(c);
-            }
-        }
-        sw.q[r] = v[0];
-        if constexpr (PK) sw.q2[r] = v[PPG - 1];
+        // 256: a row past the query (no match anywhere)
+        auto conv = [](uint32_t c) -> uint32_t {
+            return c > 255u ? 0u : PK ? query_table(query_mask_bf((unsigned char)c))
+                                      : byte_qcode((unsigned char)c);
+        };
+        sw.q[r] = conv(sw.q[r]);
+        if constexpr (PK) sw.q2[r] = conv(sw.q2[r]);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_s_waitcnt(0);  // LDS writes of this wave visible to its lanes
@@ -1392,7 +1434,7 @@ hipError_t launch_plain(const char *q, const char *r, int qlen, int rlen_max, co
 // Persistent grid of one row-class kernel: as many 256-thread blocks as fit
 // on every CU at once (cached per kernel); `waves_per_cu` > 0 caps it.
 template <int RR, bool PK, class K>
-static int full_grid(K kernel, int cu_count, int waves_per_cu) {
+static int full_grid(K kernel, int cu_count, int waves_per_cu, int max_tasks) {
     static int cached = -1;  // one per row class and lane width
     if (cached < 0) {
         int blocks = 0;
@@ -1403,10 +1445,14 @@ static int full_grid(K kernel, int cu_count, int waves_per_cu) {
     }
     int b = cached;
     if (waves_per_cu > 0) b = min(b, max(1, waves_per_cu / 4));
+    // with the task count known, no more waves than tasks: every wave of the
+    // grid takes at least one turn at the queue's atomic counter, so idle
+    // waves lengthen a small class's kernel
+    if (max_tasks > 0) return min(b * cu_count, (max_tasks + 3) / 4);
     return b * cu_count;
 }
 
-hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, hipStream_t st) {
+hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, int max_tasks, hipStream_t st) {
     auto k = sw_batch_wide_kernel<kWideRows>;
     static int blocks = -1;  // per CU, at full occupancy
     if (blocks < 0) {
@@ -1415,18 +1461,20 @@ hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, hipStream_t st) 
             b = 2;
         blocks = b;
     }
-    hipLaunchKernelGGL(k, dim3(blocks * cu_count), dim3(256), 0, st, ba);
+    int grid = blocks * cu_count;
+    if (max_tasks > 0) grid = min(grid, (max_tasks + 3) / 4);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, ba);
     return hipGetLastError();
 }
 
 hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
-                        hipStream_t st) {
+                        int max_tasks, hipStream_t st) {
     if (int16) {
         switch (rows) {
 #define SWIFT_L16(RR)                                                                      \
     case RR: {                                                                             \
         auto k = sw_batch_kernel<RR, true, kOneHot>;                                       \
-        hipLaunchKernelGGL(k, dim3(full_grid<RR, true>(k, cu_count, waves_per_cu)), dim3(256), 0, st, ba); \
+        hipLaunchKernelGGL(k, dim3(full_grid<RR, true>(k, cu_count, waves_per_cu, max_tasks)), dim3(256), 0, st, ba); \
         break;                                                                             \
     }
             SWIFT_R16_CASES(SWIFT_L16)
@@ -1438,7 +1486,7 @@ hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count,
 #define SWIFT_L32(RR)                                                                      \
     case RR: {                                                                             \
         auto k = sw_batch_kernel<RR, false, kBytes>;                                       \
-        hipLaunchKernelGGL(k, dim3(full_grid<RR, false>(k, cu_count, waves_per_cu)), dim3(256), 0, st, ba); \
+        hipLaunchKernelGGL(k, dim3(full_grid<RR, false>(k, cu_count, waves_per_cu, max_tasks)), dim3(256), 0, st, ba); \
         break;                                                                             \
     }
             SWIFT_R32_CASES(SWIFT_L32)

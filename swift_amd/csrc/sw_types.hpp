@@ -139,10 +139,12 @@ struct TaskArgs {
 };
 
 // launch helpers (sw_kernels.hip)
+// max_tasks > 0: the class's task count, known on the host (the grid then
+// has no more waves than tasks); 0: a full persistent grid
 hipError_t launch_batch(const BatchArgs &ba, bool int16, int rows, int cu_count, int waves_per_cu,
-                        hipStream_t st);
+                        int max_tasks, hipStream_t st);
 // the wide form's kernel (int16, kWideRows rows per lane)
-hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, hipStream_t st);
+hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, int max_tasks, hipStream_t st);
 // deferred walk-back over pairs order[*first .. *first + *count) (first may
 // be null: 0; max_pairs bounds *count); moves holds move_words words per pair
 // id, move_words >= (kMaxBatchQ + kMaxBatchR) / 16 + 1
