@@ -1574,6 +1574,11 @@ hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildS
 hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
     if (a.rd.nreads <= 0) return hipSuccess;
     const bool hash_first = a.hash_first && mid_blocks > 0 && a.ix.cid_ok;
+    // the tier kernels stride over strands (every one, or a list of some):
+    // no more workgroups than strands (small batches; each surplus
+    // workgroup would still clear its LDS tables)
+    const int nstr = 2 * a.rd.nreads;
+    auto grid = [nstr](int g) { return dim3((unsigned)std::max(1, std::min(g, nstr))); };
     if (!hash_first) hipLaunchKernelGGL(map_strands, dim3(2 * a.rd.nreads), dim3(64), 0, st, a);
     if (mid_blocks > 0) {
         if (a.ix.cid_ok && hash_first && a.max_len <= kShortRead) {
@@ -1583,36 +1588,36 @@ hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
             constexpr int kWpe = 8;  // waves per SIMD: kWpe / 2 workgroups of 8 waves per CU
             hipLaunchKernelGGL(
                 (map_strands_filter<512, kHashCap, 1024, 768, kShortRead, kWpe, false>),
-                dim3(kWpe / 2 * mid_blocks), dim3(512), 0, st, a, nullptr, nullptr, a.ovl_list,
+                grid(kWpe / 2 * mid_blocks), dim3(512), 0, st, a, nullptr, nullptr, a.ovl_list,
                 a.ovl_count, a.mid_list, a.mid_count);
             hipLaunchKernelGGL((map_strands_hash<512, kHashSlots, kHashCap, kShortRead, 4>),
-                               dim3(2 * mid_blocks), dim3(512), 0, st, a, a.ovl_list, a.ovl_count,
+                               grid(2 * mid_blocks), dim3(512), 0, st, a, a.ovl_list, a.ovl_count,
                                a.mid_list, a.mid_count);
             // strands past kHashCap hits (reads near the short-read limit):
             // the filter stage at 1024 threads, one workgroup per CU; what it
             // cannot take goes on to the workgroup sort
             hipLaunchKernelGGL(
                 (map_strands_filter<1024, kLongFilterCap, 2048, 1536, kShortRead, 4, true>),
-                dim3(mid_blocks / 2), dim3(1024), 0, st, a, a.mid_list, a.mid_count,
+                grid(mid_blocks / 2), dim3(1024), 0, st, a, a.mid_list, a.mid_count,
                 a.mid2_list, a.mid2_count, a.mid2_list, a.mid2_count);
-            hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), dim3(mid_blocks), dim3(1024), 0,
+            hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), grid(mid_blocks), dim3(1024), 0,
                                st, a, a.mid2_list, a.mid2_count, a.big_list, a.big_count);
             if (a.scratch_slots > 0)
-                hipLaunchKernelGGL(map_strands_big, dim3(a.scratch_slots), dim3(256), 0, st, a);
+                hipLaunchKernelGGL(map_strands_big, grid(a.scratch_slots), dim3(256), 0, st, a);
             return hipGetLastError();
         } else if (a.ix.cid_ok) {
             hipLaunchKernelGGL((map_strands_hash<512, kHashSlots, kHashCap, kMaxMapRead, 4>),
-                               dim3(2 * mid_blocks), dim3(512), 0, st, a,
+                               grid(2 * mid_blocks), dim3(512), 0, st, a,
                                hash_first ? nullptr : a.ovl_list,
                                hash_first ? nullptr : a.ovl_count, a.mid_list, a.mid_count);
         } else
-            hipLaunchKernelGGL((map_strands_wg<256, kMidCap1>), dim3(5 * mid_blocks), dim3(256),
+            hipLaunchKernelGGL((map_strands_wg<256, kMidCap1>), grid(5 * mid_blocks), dim3(256),
                                0, st, a, a.ovl_list, a.ovl_count, a.mid_list, a.mid_count);
-        hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), dim3(mid_blocks), dim3(1024), 0, st,
+        hipLaunchKernelGGL((map_strands_wg<1024, kMidCap>), grid(mid_blocks), dim3(1024), 0, st,
                            a, a.mid_list, a.mid_count, a.big_list, a.big_count);
     }
     if (a.scratch_slots > 0)
-        hipLaunchKernelGGL(map_strands_big, dim3(a.scratch_slots), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(map_strands_big, grid(a.scratch_slots), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
