@@ -860,6 +860,121 @@ __global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t
     }
 }
 
+// Deferred walk of one pair by a whole wave, for batches too small to fill
+// the chip (a lane per pair leaves each wave waiting on one dependent
+// granule load after another, and every lane's wait is the wave's).  The
+// walk itself is wave-uniform and runs in scalar registers; the trace
+// around it is loaded as a window of granules, one per lane -- lane groups
+// (g0 - NG, g0], every trace word, step blocks (tb0 - NT, tb0] -- and each
+// cell's code read from its lane with v_readlane.  The walk only moves up
+// and left, so it stays in a window until its lane group or step block
+// passes the window's lower edge; a new window is then loaded at the
+// current cell, one load per lane.  Same path, moves and results as
+// walk_cells.  tr: the pair's lane group at step 0, word 0.
+template <int NMV>
+__device__ __forceinline__ WalkOut walk_wave(const uint4 *tr, int R, int W, int kl, int bitb,
+                                             int score, int row, int col, int qlen, int rlen,
+                                             uint32_t *mv) {
+    const int lane = threadIdx.x & 63;
+    int start = 0, nrb = -1, tick = 0;
+    if (!(score > 0 && row >= 1 && row <= qlen && col >= 1 && col <= rlen))
+        return WalkOut{start, start + nrb, tick};
+    const uint32_t inv = (65536u + (uint32_t)R - 1) / (uint32_t)R;  // x / R for x < 2048
+    // window shape: about 24 rows of lane groups, the rest of the 64 lanes
+    // in step blocks; lanes past NG x W x NT load a copy of a window granule
+    const int NG = min(8, (24 + R - 1) / R + 1);
+    const int WT = 64 / NG, NT = WT / W;
+    const int dg = min(lane / WT, NG - 1), r0 = lane - (lane / WT) * WT;
+    const int lw = min(r0 / NT, W - 1), dtb = r0 - (r0 / NT) * NT;
+    const int kmask = (1 << kl) - 1;
+    int g0 = -1, tb0 = 0;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    // the moves, 16 per word, word m * 64 + lane in mvr[m] (stored at the end:
+    // a store inside the walk would be waited for with the next window load)
+    uint32_t mvr[NMV];
+#pragma unroll
+    for (int m = 0; m < NMV; ++m) mvr[m] = 0;
+    uint32_t mword = 0;
+    int ci = row, cj = col;
+    for (;;) {
+        const int g = (int)(((uint32_t)(ci - 1) * inv) >> 16), k = (ci - 1) - g * R;
+        const int t = (cj - 1) + g, tb = t >> 2, w = k >> kl;
+        if (g0 < 0 || g0 - g >= NG || tb0 - tb >= NT) {
+            g0 = g;
+            tb0 = tb;
+            v = tr[(max(tb0 - dtb, 0) * W + lw) * 64 + max(g0 - dg, 0)];
+        }
+        const int L = (g0 - g) * WT + w * NT + (tb0 - tb);
+        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)v.x, L);
+        const uint32_t y = (uint32_t)__builtin_amdgcn_readlane((int)v.y, L);
+        const uint32_t z = (uint32_t)__builtin_amdgcn_readlane((int)v.z, L);
+        const uint32_t u = (uint32_t)__builtin_amdgcn_readlane((int)v.w, L);
+        const uint32_t word = (t & 2) ? ((t & 1) ? u : z) : ((t & 1) ? y : x);
+        const int c = (int)((word >> (bitb + 2 * (k & kmask))) & 3u);
+        const int ni = (c & 2) ? ci - 1 : ci;             // diag (3), up (2)
+        const int nj = (c == 3 || c == 1) ? cj - 1 : cj;  // diag (3), left (1)
+        if (ni == ci && nj == cj) break;                  // a start pointer
+        const int op = nj == cj ? 1 : ni == ci ? 2 : 0;
+        if (op != 1) ++nrb;
+        mword |= (uint32_t)op << (2 * (tick & 15));
+        if ((tick & 15) == 15) {
+#pragma unroll
+            for (int m = 0; m < NMV; ++m)
+                if ((tick >> 4) == m * 64 + lane) mvr[m] = mword;
+            mword = 0;
+        }
+        start = cj;
+        ++tick;
+        if (ni == 0 || nj == 0) break;  // the target is in row/column 0
+        ci = ni;
+        cj = nj;
+    }
+    if (tick & 15) {
+#pragma unroll
+        for (int m = 0; m < NMV; ++m)
+            if ((tick >> 4) == m * 64 + lane) mvr[m] = mword;
+    }
+    const int nwords = (tick + 15) >> 4;
+#pragma unroll
+    for (int m = 0; m < NMV; ++m)
+        if (m * 64 + lane < nwords) mv[m * 64 + lane] = mvr[m];
+    return WalkOut{start, start + nrb, tick};
+}
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// sw_walk_kernel's work, one wave per pair (small batches); a path's moves
+// fit one word per lane
+static_assert((kMaxBatchQ + kMaxBatchR) / 16 + 1 <= 64, "moves of a batched pair");
+__global__ void __launch_bounds__(256) sw_walk_wave_kernel(BatchArgs a, const int32_t *order,
+                                                           const int32_t *first_p,
+                                                           const int32_t *count, uint32_t *moves,
+                                                           int move_words) {
+    const int n = uni(*count);
+    const int32_t *ord = order + (first_p ? uni(*first_p) : 0);
+    const int nw = gridDim.x * 4;
+    for (int i = uni((int)(blockIdx.x * 4 + (threadIdx.x >> 6))); i < n; i += nw) {
+        const int p = uni(ord[i]);
+        const PairResult r = a.res[p];
+        if (uni(r.flags) & 1) continue;
+        const PairTrace pt = a.ptr[p];
+        const int half = uni(pt.half), R = uni(pt.rows), W = uni(pt.words);
+        const int64_t base = ((int64_t)uni((int)(pt.base >> 32)) << 32) |
+                             (uint32_t)uni((int)(uint32_t)pt.base);
+        const uint4 *tr = reinterpret_cast<const uint4 *>(a.trace) + base / 4 + uni(pt.lane0);
+        const int score = uni(r.score);
+        WalkOut wo = walk_wave<1>(tr, R, W, (half & 2) ? 3 : 4, (half & 1) * 16, score, uni(r.row),
+                               uni(r.col), uni(a.qlen[p]), uni(a.rlen[p]),
+                               moves + (int64_t)p * move_words);
+        if ((threadIdx.x & 63) == 0) {
+            a.score_out[p] = score;
+            a.start_out[p] = wo.start;
+            a.end_out[p] = wo.end;
+            a.alen_out[p] = wo.len;
+        }
+    }
+}
+
 // One pair's consensus expansion, by one 16-lane row of a wave (lane sl of
 // the row).  A pair's moves are taken in chunks of 256 (16 per lane); a
 // prefix sum of the row and column steps over the row's 16 lanes gives each
@@ -1017,6 +1132,29 @@ __global__ void __launch_bounds__(256) sw_plain_walk_kernel(const uint32_t *trac
     out.start[gp] = wo.start;
     out.end[gp] = wo.end;
     out.alen[gp] = wo.len;
+}
+
+// sw_plain_walk_kernel's work, one wave per pair (small calls); NMV: move
+// words per lane (a path's moves fit NMV * 64 words)
+template <int NMV>
+__global__ void __launch_bounds__(256)
+sw_plain_walk_wave_kernel(const uint32_t *trace, int64_t wpp, int R, int W, int qlen, int npairs,
+                          int pair0, const int4 *res, uint32_t *moves, int move_words,
+                          PlainOut out) {
+    const int p = uni((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (p >= npairs) return;
+    const int4 r = res[p];
+    const int score = uni(r.x);
+    const uint4 *tr = reinterpret_cast<const uint4 *>(trace) + (int64_t)p * wpp / 4;
+    WalkOut wo = walk_wave<NMV>(tr, R, W, 4, 0, score, uni(r.y), uni(r.z), qlen, uni(r.w),
+                                moves + (int64_t)p * move_words);
+    if ((threadIdx.x & 63) == 0) {
+        const int gp = pair0 + p;
+        out.score[gp] = (float)score;
+        out.start[gp] = wo.start;
+        out.end[gp] = wo.end;
+        out.alen[gp] = wo.len;
+    }
 }
 
 // Plain path: the consensus strings from the walks' moves, four pairs per
@@ -1412,9 +1550,20 @@ static hipError_t launch_plain_walk(const char *q, const char *r, int qlen, int 
     if (npairs <= 0) return hipSuccess;
     const int R = plain_rows_for(qlen);
     const PlainScratch ps(qlen, rlen_max, npairs, scratch);
-    hipLaunchKernelGGL(sw_plain_walk_kernel, dim3((npairs + 255) / 256), dim3(256), 0, st,
-                       ps.trace, ps.wpp, R, plain_trace_words(R), qlen, npairs, pair0, ps.res,
-                       ps.moves, ps.mw, out);
+    const int nmv = (ps.mw + 63) / 64;
+    if (npairs < kWaveWalkPairs && nmv <= 5) {
+        auto k = nmv <= 1 ? sw_plain_walk_wave_kernel<1>
+                 : nmv == 2 ? sw_plain_walk_wave_kernel<2>
+                 : nmv == 3 ? sw_plain_walk_wave_kernel<3>
+                 : nmv == 4 ? sw_plain_walk_wave_kernel<4>
+                            : sw_plain_walk_wave_kernel<5>;
+        hipLaunchKernelGGL(k, dim3((npairs + 3) / 4), dim3(256), 0, st, ps.trace, ps.wpp, R,
+                           plain_trace_words(R), qlen, npairs, pair0, ps.res, ps.moves, ps.mw, out);
+    } else {
+        hipLaunchKernelGGL(sw_plain_walk_kernel, dim3((npairs + 255) / 256), dim3(256), 0, st,
+                           ps.trace, ps.wpp, R, plain_trace_words(R), qlen, npairs, pair0, ps.res,
+                           ps.moves, ps.mw, out);
+    }
     const int qw = (qlen + 6) / 4, rw = (rlen_max + 6) / 4;
     hipLaunchKernelGGL(sw_plain_expand_kernel, dim3((npairs + 3) / 4), dim3(64),
                        sizeof(uint32_t) * 4 * (size_t)(qw + rw), st, q, r, qlen, rlen_max, npairs,
@@ -1501,8 +1650,12 @@ hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t 
                        const int32_t *count, int max_pairs, uint32_t *moves, int move_words,
                        hipStream_t st) {
     if (max_pairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sw_walk_kernel, dim3((max_pairs + 255) / 256), dim3(256), 0, st, ba, order,
-                       first, count, moves, move_words);
+    if (max_pairs < kWaveWalkPairs)
+        hipLaunchKernelGGL(sw_walk_wave_kernel, dim3((max_pairs + 3) / 4), dim3(256), 0, st, ba,
+                           order, first, count, moves, move_words);
+    else
+        hipLaunchKernelGGL(sw_walk_kernel, dim3((max_pairs + 255) / 256), dim3(256), 0, st, ba,
+                           order, first, count, moves, move_words);
     hipLaunchKernelGGL(sw_expand_kernel, dim3((max_pairs + 15) / 16), dim3(256), 0, st, ba,
                        order, first, count, moves, move_words);
     return hipGetLastError();
