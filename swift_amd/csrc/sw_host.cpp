@@ -454,12 +454,15 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     // the smallest batches with host-known sizes round to kCoarseRowsSmall,
     // or under kCoarseTinyPairs to a single class: their class kernels run
     // about one task each, one after another, so fewer classes is lower
-    // latency
+    // latency.  From kWideShortPairs on, the single (wide) class runs two of
+    // its waves per SIMD; a batch whose queries all fit the 10-row class
+    // (maxq <= 160) is then faster in that one 16-lane class.
+    const bool short_q = io.maxq <= kBatchG * kCoarseRowsSmall;
     const int coarse =
         cmode == 1 || (cmode < 0 && (w.nside == 0 || !sync_sizes || n < kCoarsePairs))
-            ? (!sync_sizes || n >= kCoarseSmallPairs ? kCoarseRows
-               : n >= kCoarseTinyPairs               ? kCoarseRowsSmall
-                                                     : kRBins)
+            ? (!sync_sizes || n >= kCoarseSmallPairs                     ? kCoarseRows
+               : n >= kCoarseTinyPairs || (n >= kWideShortPairs && short_q) ? kCoarseRowsSmall
+                                                                           : kRBins)
             : 0;
     // a single class (the smallest batches) runs the wide form: its few
     // tasks are each one wave, and a wave's sweep is the batch's latency

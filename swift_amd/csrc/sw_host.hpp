@@ -124,6 +124,9 @@ struct Tuning {
 constexpr int kCoarsePairs = 131072;
 constexpr int kCoarseSmallPairs = 16384;  // ... rounded to kCoarseRowsSmall below this
 constexpr int kCoarseTinyPairs = 4096;    // ... and to one class (R = kRBins) below this
+// ... but from this many pairs only when some query needs more than
+// kCoarseRowsSmall rows per 16-lane group
+constexpr int kWideShortPairs = 2048;
 Tuning &tuning();
 
 }  // namespace swift

@@ -141,6 +141,19 @@ def test_batch_uneven_vs_oracle(A, orc, coarse):
         compare(res.record(i), orc.sw(rs[i], qs[i]), i)
 
 
+def test_batch_short_queries_mid_size(A, orc):
+    """3,000 pairs whose queries all fit the 10-row class (<= 160 bases):
+    one 16-lane class instead of the wide form, same results."""
+    rnd = random.Random(31)
+    qs, rs = rand_pairs(rnd, 3000, 160, 400, [b"ACGT", b"ACGTN"])
+    res = A.sw_batch(qs, rs)
+    assert A.batch_stats()["ntasks16"] <= 3000 // 8 + 20  # 8 pairs per task, not 2
+    sc, st, en, al, qc, rc = orc.sw_batch(qs, rs, nthreads=8)
+    assert np.array_equal(res.score, sc) and np.array_equal(res.start, st)
+    assert np.array_equal(res.end, en) and np.array_equal(res.length, al)
+    assert res.qry_cons == qc and res.ref_cons == rc
+
+
 def test_batch_wide_form_limits(A, orc):
     """The wide form at its limits: 4,095 pairs (the largest batch that takes
     it) with the longest queries and references (320 x 480, 543 steps), and
