@@ -1445,37 +1445,54 @@ __global__ void __launch_bounds__(256) choose_best(PairsDev p, int nreads, const
 }
 
 // ---------------------------------------------------------------------------
-// Consensus strings of records [0, *n) packed back to back (record i's
-// alen[i] bytes at pos[i], an exclusive prefix of alen): one wave per record.
+// Serving readback
 // ---------------------------------------------------------------------------
 struct LoadLen64 {
     const int32_t *p;
     __device__ int64_t operator()(int i) const { return p[i]; }
 };
 
-__global__ void __launch_bounds__(256) pack_cons(const int32_t *n_dev, const int32_t *alen,
-                                                 const int64_t *pos, const char *qc,
-                                                 const char *rc, int64_t cs, char *oq, char *orr) {
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= *n_dev) return;
+// One served batch straight into pinned host memory (serve_layout): the
+// columns and distances with consecutive threads on consecutive entries (so
+// the writes across the bus are whole lines), then the strings, one wave per
+// record.  One kernel and one synchronisation instead of a copy per array.
+__global__ void __launch_bounds__(256) serve_out(ServeOut o) {
+    const int n = *o.nrec;
+    const int64_t packed = n > 0 ? *o.total : 0;
+    const ServeLayout l = serve_layout(n, packed);
+    const int gt = blockIdx.x * 256 + (int)threadIdx.x, G = gridDim.x * 256;
+    if (gt == 0) {
+        *reinterpret_cast<int32_t *>(o.out) = n;
+        *reinterpret_cast<int64_t *>(o.out + 8) = packed;
+        reinterpret_cast<int32_t *>(o.out + 16)[0] = o.flags[0];
+        reinterpret_cast<int32_t *>(o.out + 16)[1] = o.flags[1];
+    }
+    int32_t *cols = reinterpret_cast<int32_t *>(o.out + l.o_col);
+#pragma unroll
+    for (int c = 0; c < kServeCols; ++c)
+        for (int k = gt; k < n; k += G) cols[(int64_t)c * n + k] = o.col[c][k];
+    int64_t *rd = reinterpret_cast<int64_t *>(o.out + l.o_rd);
+    for (int k = gt; k < n; k += G) rd[k] = o.rdist[k];
     const int lane = threadIdx.x & 63;
-    const int L = alen[i];
-    const char *q = qc + (int64_t)i * cs, *r = rc + (int64_t)i * cs;
-    char *dq = oq + pos[i], *dr = orr + pos[i];
-    for (int k = lane; k < L; k += 64) {
-        dq[k] = q[k];
-        dr[k] = r[k];
+    const int32_t *alen = o.col[o.alen_col];
+    char *oq = o.out + l.o_q, *orr = o.out + l.o_r;
+    for (int i = gt >> 6; i < n; i += G >> 6) {
+        const int L = alen[i];
+        const char *q = o.qc + (int64_t)i * o.cs, *r = o.rc + (int64_t)i * o.cs;
+        char *dq = oq + o.pos[i], *dr = orr + o.pos[i];
+        for (int k = lane; k < L; k += 64) {
+            dq[k] = q[k];
+            dr[k] = r[k];
+        }
     }
 }
 
-hipError_t launch_pack_cons(const int32_t *n_dev, int cap, const int32_t *alen, int64_t *pos,
-                            int64_t *sums, int64_t *total, const char *qc, const char *rc,
-                            int64_t cs, char *oq, char *orr, hipStream_t st) {
-    hipError_t e = launch_scan<int64_t>(LoadLen64{alen}, StoreArr<int64_t>{pos}, n_dev, cap, sums,
-                                        nullptr, total, nullptr, st);
-    if (e != hipSuccess || cap <= 0) return e;
-    hipLaunchKernelGGL(pack_cons, dim3((cap + 3) / 4), dim3(256), 0, st, n_dev, alen, pos, qc, rc,
-                       cs, oq, orr);
+hipError_t launch_serve_out(const ServeOut &o, int cap, hipStream_t st) {
+    hipError_t e = launch_scan<int64_t>(LoadLen64{o.col[o.alen_col]}, StoreArr<int64_t>{o.pos},
+                                        o.nrec, cap, o.sums, nullptr, o.total, nullptr, st);
+    if (e != hipSuccess) return e;
+    const int blocks = std::max(1, std::min((cap + 3) / 4, 1024));
+    hipLaunchKernelGGL(serve_out, dim3(blocks), dim3(256), 0, st, o);
     return hipGetLastError();
 }
 

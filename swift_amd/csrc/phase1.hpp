@@ -134,12 +134,38 @@ hipError_t launch_records(const MapArgs &m, const PairsDev &p, int before, int w
                           hipStream_t st);
 hipError_t launch_choose_best(const PairsDev &p, int nreads, const int32_t *score,
                               int32_t *keep, int32_t *ntied, hipStream_t st);
-// Packs records' consensus strings (stride cs) back to back: pos[i] =
-// exclusive prefix of alen over [0, *n_dev), *total = sum; sums holds
-// cap / 2048 + 2 values.
-hipError_t launch_pack_cons(const int32_t *n_dev, int cap, const int32_t *alen, int64_t *pos,
-                            int64_t *sums, int64_t *total, const char *qc, const char *rc,
-                            int64_t cs, char *oq, char *orr, hipStream_t st);
+// A served batch's results in host-visible (pinned) memory, written by one
+// kernel: a 64-byte header (int32 records at 0, int64 packed string bytes at
+// 8, the two int32 mapping flags at 16), then kServeCols int32 record
+// columns of n entries each, the n int64 distances, and the query and
+// reference consensus strings packed back to back (record i's alen bytes at
+// the exclusive prefix of alen).
+constexpr int kServeCols = 12;
+struct ServeLayout {
+    int64_t o_col, o_rd, o_q, o_r, end;
+};
+__host__ __device__ inline ServeLayout serve_layout(int n, int64_t packed) {
+    ServeLayout l;
+    l.o_col = 64;
+    l.o_rd = l.o_col + (((int64_t)kServeCols * 4 * n + 7) & ~(int64_t)7);
+    l.o_q = l.o_rd + 8 * (int64_t)n;
+    l.o_r = l.o_q + ((packed + 7) & ~(int64_t)7);
+    l.end = l.o_r + packed;
+    return l;
+}
+struct ServeOut {
+    const int32_t *nrec;                // records (device)
+    const int32_t *flags;               // mapping flags, 2 ints (device)
+    const int32_t *col[kServeCols];     // record columns; col[alen_col] the lengths
+    int alen_col;
+    const int64_t *rdist;
+    const char *qc, *rc;                // consensus strings at stride cs
+    int64_t cs;
+    int64_t *pos, *sums, *total;        // scan scratch: cap + 1, cap / 2048 + 4, 1
+    char *out;                          // pinned host memory, serve_layout(cap, cap * cs).end
+};
+// the scan of the string lengths, then the kernel that writes everything
+hipError_t launch_serve_out(const ServeOut &o, int cap, hipStream_t st);
 // mapHits6GetBestHits over caller hits already on the device: sorts the
 // triples in place (refIdx, shift, refPos ascending) and writes the best
 // clusters' first hits.  kbuf holds next_pow2(n) 16-byte keys.

@@ -75,7 +75,7 @@ struct SrvSlot {
     Workspace *sws = nullptr;
     MapWork mw;
     hipStream_t st = nullptr;
-    DevArr seq, off, len, nrec, rdist, qc, rc, pos, sums, pq, pr;
+    DevArr inb, nrec, rdist, qc, rc, pos, sums;
     DevArr col[kRecI32];
     PinBuf pin_in, pin_out;
 };
@@ -95,13 +95,15 @@ struct Server {
     int64_t completed = 0, batches = 0, batched_reads = 0, max_coalesced = 0;
 };
 
+static_assert(kRecI32 == kServeCols, "record columns of the serving readback");
+
 static int64_t cstride_for(int max_len) { return ((int64_t)max_len + 302 + 3) & ~3; }
 
 // device bytes one slot needs for a batch of nreads reads of <= max_len
 static int64_t slot_bytes(int nreads, int max_len, int max_hits) {
     const int64_t cap = 2 * (int64_t)nreads * std::max(max_hits, 1);
     return align_bytes_estimate(nreads, max_len, max_hits) + (int64_t)nreads * (max_len + 12) +
-           cap * (kRecI32 * 4 + 16 + 4 * cstride_for(max_len));
+           cap * (kRecI32 * 4 + 16 + 2 * cstride_for(max_len));
 }
 
 static size_t align8(size_t x) { return (x + 7) & ~(size_t)7; }
@@ -135,20 +137,16 @@ static int run_batch(Server &S, SrvSlot &sl, std::vector<SrvRequest *> &batch) {
     }
     const int64_t cap = 2 * (int64_t)nreads * S.max_hits;
     const int64_t cs = cstride_for(max_len);
-    if ((rc = ensure(sl.seq, (size_t)nbases + 64)) || (rc = ensure(sl.off, 8 * (size_t)nreads + 64)) ||
-        (rc = ensure(sl.len, 4 * (size_t)nreads + 64)) || (rc = ensure(sl.nrec, 64)) ||
+    const size_t in_bytes = o_len + 4 * (size_t)nreads;
+    if ((rc = ensure(sl.inb, in_bytes + 64)) || (rc = ensure(sl.nrec, 64)) ||
         (rc = ensure(sl.rdist, 8 * (size_t)cap + 64)) || (rc = ensure(sl.qc, (size_t)cap * cs + 64)) ||
-        (rc = ensure(sl.rc, (size_t)cap * cs + 64)) || (rc = ensure(sl.pq, (size_t)cap * cs + 64)) ||
-        (rc = ensure(sl.pr, (size_t)cap * cs + 64)) || (rc = ensure(sl.pos, 8 * (size_t)cap + 64)) ||
+        (rc = ensure(sl.rc, (size_t)cap * cs + 64)) || (rc = ensure(sl.pos, 8 * (size_t)cap + 64)) ||
         (rc = ensure(sl.sums, 8 * ((size_t)cap / 2048 + 4))))
         return rc;
     for (int c = 0; c < kRecI32; ++c)
         if ((rc = ensure(sl.col[c], 4 * (size_t)cap + 64))) return rc;
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.seq.ptr, sl.pin_in.p, (size_t)nbases, hipMemcpyHostToDevice, sl.st));
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.off.ptr, sl.pin_in.p + o_off, 8 * (size_t)nreads,
-                                 hipMemcpyHostToDevice, sl.st));
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.len.ptr, sl.pin_in.p + o_len, 4 * (size_t)nreads,
-                                 hipMemcpyHostToDevice, sl.st));
+    // the staged reads in one copy: bases | offsets | lengths
+    SWIFT_HIP_TRY(hipMemcpyAsync(sl.inb.ptr, sl.pin_in.p, in_bytes, hipMemcpyHostToDevice, sl.st));
     swift_records rec;
     rec.nrecords = sl.nrec.as<int32_t>();
     int32_t **cols[kRecI32] = {&rec.read, &rec.strand, &rec.ref_idx, &rec.shift, &rec.ref_pos,
@@ -160,7 +158,9 @@ static int run_batch(Server &S, SrvSlot &sl, std::vector<SrvRequest *> &batch) {
     rec.ref_cons = sl.rc.as<char>();
     rec.cons_stride = cs;
     rec.capacity = (int32_t)cap;
-    ReadsDev rd{sl.seq.as<char>(), sl.off.as<int64_t>(), sl.len.as<int32_t>(), nreads};
+    char *inb = sl.inb.as<char>();
+    ReadsDev rd{inb, reinterpret_cast<const int64_t *>(inb + o_off),
+                reinterpret_cast<const int32_t *>(inb + o_len), nreads};
     // the pipeline without its own final readback: the mapping error flags
     // come back with the record count below
     MapArgs m;
@@ -168,14 +168,27 @@ static int run_batch(Server &S, SrvSlot &sl, std::vector<SrvRequest *> &batch) {
     if ((rc = align_prepare(sl.mw, *S.ix, rd, max_len, S.max_hits, rec, &m, &p)) ||
         (rc = align_enqueue(m, p, sl.sws, nreads, max_len, S.sc, rec, sl.st, true)))
         return rc;
-    // consensus strings packed on the device, then one exact-size copy each
-    int64_t *total_d = sl.nrec.as<int64_t>() + 1;
-    SWIFT_HIP_TRY(launch_pack_cons(rec.nrecords, (int)cap, rec.alen, sl.pos.as<int64_t>(),
-                                   sl.sums.as<int64_t>(), total_d, rec.qry_cons, rec.ref_cons, cs,
-                                   sl.pq.as<char>(), sl.pr.as<char>(), sl.st));
-    if ((rc = sl.pin_out.ensure(64))) return rc;
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p, sl.nrec.ptr, 16, hipMemcpyDeviceToHost, sl.st));
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p + 16, m.ovl_count, 8, hipMemcpyDeviceToHost, sl.st));
+    // the records, distances, packed strings, count and mapping flags
+    // written by one kernel into pinned memory sized for the worst case; one
+    // synchronisation brings the batch back
+    const ServeLayout worst = serve_layout((int)cap, cap * cs);
+    if ((rc = sl.pin_out.ensure((size_t)worst.end + 64))) return rc;
+    {
+        ServeOut so;
+        so.nrec = rec.nrecords;
+        so.flags = m.ovl_count;
+        for (int c = 0; c < kRecI32; ++c) so.col[c] = *cols[c];
+        so.alen_col = 9;
+        so.rdist = rec.ref_distance;
+        so.qc = rec.qry_cons;
+        so.rc = rec.ref_cons;
+        so.cs = cs;
+        so.pos = sl.pos.as<int64_t>();
+        so.sums = sl.sums.as<int64_t>();
+        so.total = sl.nrec.as<int64_t>() + 1;
+        so.out = sl.pin_out.p;
+        SWIFT_HIP_TRY(launch_serve_out(so, (int)cap, sl.st));
+    }
     SWIFT_HIP_TRY(hipStreamSynchronize(sl.st));
     int32_t n, mflags[2];
     int64_t packed;
@@ -183,23 +196,12 @@ static int run_batch(Server &S, SrvSlot &sl, std::vector<SrvRequest *> &batch) {
     memcpy(&packed, sl.pin_out.p + 8, 8);
     memcpy(mflags, sl.pin_out.p + 16, 8);
     if ((rc = map_error(mflags[1]))) return rc;
-    if (n == 0) packed = 0;
-    const size_t o_col = 0, o_rd = align8(kRecI32 * 4 * (size_t)n), o_q = o_rd + 8 * (size_t)n,
-                 o_r = o_q + align8((size_t)packed), o_end = o_r + (size_t)packed;
-    if ((rc = sl.pin_out.ensure(o_end + 64))) return rc;
-    for (int c = 0; c < kRecI32; ++c)
-        SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p + o_col + (size_t)c * 4 * n, sl.col[c].ptr,
-                                     4 * (size_t)n, hipMemcpyDeviceToHost, sl.st));
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p + o_rd, sl.rdist.ptr, 8 * (size_t)n,
-                                 hipMemcpyDeviceToHost, sl.st));
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p + o_q, sl.pq.ptr, (size_t)packed, hipMemcpyDeviceToHost, sl.st));
-    SWIFT_HIP_TRY(hipMemcpyAsync(sl.pin_out.p + o_r, sl.pr.ptr, (size_t)packed, hipMemcpyDeviceToHost, sl.st));
-    SWIFT_HIP_TRY(hipStreamSynchronize(sl.st));
+    const ServeLayout lay = serve_layout(n, packed);
     const int32_t *col[kRecI32];
     for (int c = 0; c < kRecI32; ++c)
-        col[c] = reinterpret_cast<const int32_t *>(sl.pin_out.p + o_col + (size_t)c * 4 * n);
-    const int64_t *hd = reinterpret_cast<const int64_t *>(sl.pin_out.p + o_rd);
-    const char *hq = sl.pin_out.p + o_q, *hr = sl.pin_out.p + o_r;
+        col[c] = reinterpret_cast<const int32_t *>(sl.pin_out.p + lay.o_col + (size_t)c * 4 * n);
+    const int64_t *hd = reinterpret_cast<const int64_t *>(sl.pin_out.p + lay.o_rd);
+    const char *hq = sl.pin_out.p + lay.o_q, *hr = sl.pin_out.p + lay.o_r;
     // split by request: records are in read order, requests are consecutive
     int r0 = 0, i = 0;
     int64_t p0 = 0;
