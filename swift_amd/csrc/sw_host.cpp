@@ -507,9 +507,18 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     BinArgs b1;
     TaskArgs t1;
     pass_args(0, nullptr, force32 ? 1 : 0, b1, t1);
-    SWIFT_HIP_TRY(launch_binning(b1, t1, w.order.as<int32_t>(), w.tasks.as<Task>(),
-                                 w.task_words.as<int64_t>(), nullptr, tot + 0,
-                                 w.scan_sums.as<int64_t>(), mt, st));
+    // the smallest batches with host-known sizes: one workgroup bins them and
+    // writes the sizes into the staging block (no gather)
+    const bool tiny_bin = sync_sizes && n <= kBinTinyPairs;
+    if (tiny_bin) {
+        const TinyStage hs{&w.stage->words, w.stage->steps, w.stage->seg, w.stage->cnt};
+        SWIFT_HIP_TRY(launch_binning_tiny(b1, t1, w.order.as<int32_t>(), w.tasks.as<Task>(),
+                                          w.task_words.as<int64_t>(), tot + 0, cnt, hs, st));
+    } else {
+        SWIFT_HIP_TRY(launch_binning(b1, t1, w.order.as<int32_t>(), w.tasks.as<Task>(),
+                                     w.task_words.as<int64_t>(), nullptr, tot + 0,
+                                     w.scan_sums.as<int64_t>(), mt, st));
+    }
     int64_t words1 = 0;
     int32_t seg1[2 * kRBins + 1], seg2[2 * kRBins + 1];
     int64_t steps1[2 * kRBins], steps2[2 * kRBins];
@@ -546,9 +555,15 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     };
     if (sync_sizes) {
         // and the error flag, in the same round trip
-        if ((rc = read_sizes(tot + 0, t1.seg_task0, tot + kTotSteps, cnt, 16, &words1, seg1,
-                             steps1)))
+        if (tiny_bin) {
+            SWIFT_HIP_TRY(hipStreamSynchronize(st));
+            words1 = w.stage->words;
+            std::memcpy(seg1, w.stage->seg, sizeof(w.stage->seg));
+            std::memcpy(steps1, w.stage->steps, sizeof(w.stage->steps));
+        } else if ((rc = read_sizes(tot + 0, t1.seg_task0, tot + kTotSteps, cnt, 16, &words1, seg1,
+                                    steps1))) {
             return rc;
+        }
         if (w.stage->cnt[kCntError]) {
             set_error("pair lengths beyond maxq/maxr or the batched limits (%d/%d)", kMaxBatchQ,
                       kMaxBatchR);

@@ -1447,6 +1447,143 @@ sw_build_tasks(const int32_t *seg_task0, const int32_t *seg_first, const int32_t
         atomicAdd(&seg_steps[threadIdx.x], steps_acc[threadIdx.x]);
 }
 
+// Exclusive scan over one 1,024-thread workgroup (each thread's value v):
+// wave scans by shuffles, then the 16 wave totals through LDS.
+template <class T>
+__device__ __forceinline__ T wg_excl_scan(T v, T *wsum, T *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(x, d, 64);
+        if (lane >= d) x += o;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    T before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const T t = wsum[i];
+        before += i < w ? t : T(0);
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// The whole of pass-1 binning for a batch of up to kBinTinyPairs pairs in
+// one workgroup (classify, bin counts, their scan, the segments, the
+// scatter, the tasks and their trace bases), with the sizes the host reads
+// written straight into its mapped staging block: one launch and no gather
+// instead of seven launches.  Same outputs as launch_binning + the gather
+// (the order within a bin may differ; nothing depends on it).
+__global__ void __launch_bounds__(1024)
+sw_bin_tiny(BinArgs a, TaskArgs ta, int32_t *order, Task *tasks, int64_t *task_words,
+            int64_t *total_words, const int32_t *counters, TinyStage hs) {
+    constexpr int PP = kBinTinyPairs / 1024;           // pairs per thread
+    constexpr int BP = (kNumBins + 1023) / 1024;       // bins per thread
+    constexpr int TP = (kBinTinyPairs / kSlotsWide + 2 * kRBins + 1023) / 1024;  // tasks per thread
+    __shared__ int32_t hist[kNumBins];  // counts, then cursors
+    __shared__ int32_t s_task0[2 * kRBins + 1], s_first[2 * kRBins], s_len[2 * kRBins];
+    __shared__ unsigned long long steps_acc[2 * kRBins];
+    __shared__ int32_t wsum32[16];
+    __shared__ int64_t wsum64[16];
+    const int tid = threadIdx.x, n = a.npairs;
+    for (int i = tid; i < kNumBins; i += 1024) hist[i] = 0;
+    if (tid < 2 * kRBins) steps_acc[tid] = 0;
+    __syncthreads();
+    int bin[PP], rank[PP];
+#pragma unroll
+    for (int k = 0; k < PP; ++k) {
+        const int p = k * 1024 + tid;
+        bin[k] = p < n ? classify_pair(a, p) : -1;
+        rank[k] = bin[k] >= 0 ? atomicAdd(&hist[bin[k]], 1) : 0;
+    }
+    __syncthreads();
+    if (tid < 2 * kRBins) {
+        int32_t len = 0;
+        for (int lb = 0; lb < kLenBins; ++lb) len += hist[tid * kLenBins + lb];
+        s_len[tid] = len;
+    }
+    // bin cursors: exclusive scan of the counts
+    int32_t loc[BP], sum = 0;
+#pragma unroll
+    for (int i = 0; i < BP; ++i) {
+        const int b = tid * BP + i;
+        loc[i] = b < kNumBins ? hist[b] : 0;
+        sum += loc[i];
+    }
+    int32_t binned;
+    int32_t run = wg_excl_scan<int32_t>(sum, wsum32, &binned);
+#pragma unroll
+    for (int i = 0; i < BP; ++i) {
+        const int b = tid * BP + i;
+        if (b < kNumBins) hist[b] = run;
+        run += loc[i];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int32_t t0 = 0;
+        for (int seg = 0; seg < 2 * kRBins; ++seg) {
+            const int cls = seg / kRBins;
+            const int slots = cls == 0 ? (ta.wide && seg == 0 ? kSlotsWide : kSlots16) : kSlots32;
+            s_first[seg] = hist[seg * kLenBins];
+            s_task0[seg] = t0;
+            t0 += (s_len[seg] + slots - 1) / slots;
+        }
+        s_task0[2 * kRBins] = t0;
+    }
+#pragma unroll
+    for (int k = 0; k < PP; ++k)
+        if (bin[k] >= 0) order[hist[bin[k]] + rank[k]] = k * 1024 + tid;
+    __syncthreads();  // the order and the segments, for every thread of the group
+    const int nt = s_task0[2 * kRBins];
+    if (tid <= 2 * kRBins) ta.seg_task0[tid] = s_task0[tid];
+    if (tid < 2 * kRBins) {
+        ta.seg_first[tid] = s_first[tid];
+        ta.seg_len[tid] = s_len[tid];
+    }
+    if (tid == 0) {
+        *ta.ntasks = nt;
+        *ta.npairs_binned = binned;
+    }
+    for (int t = tid; t < nt; t += 1024)
+        build_task(t, s_task0, s_first, s_len, order, a.rlen, tasks, task_words, steps_acc,
+                   ta.wide);
+    __syncthreads();  // the task words
+    int64_t tw[TP], tsum = 0;
+#pragma unroll
+    for (int i = 0; i < TP; ++i) {
+        const int t = tid * TP + i;
+        tw[i] = t < nt ? task_words[t] : 0;
+        tsum += tw[i];
+    }
+    int64_t words;
+    int64_t base = wg_excl_scan<int64_t>(tsum, wsum64, &words);
+#pragma unroll
+    for (int i = 0; i < TP; ++i) {
+        const int t = tid * TP + i;
+        if (t < nt) tasks[t].trace_base = base;
+        base += tw[i];
+    }
+    if (tid < 2 * kRBins) {
+        ta.seg_steps[tid] = steps_acc[tid];
+        hs.steps[tid] = (int64_t)steps_acc[tid];
+    }
+    if (tid <= 2 * kRBins) hs.seg[tid] = s_task0[tid];
+    if (tid == 0) {
+        *total_words = words;
+        *hs.words = words;
+    }
+    __syncthreads();  // the counters (the error flag of classify_pair included)
+    if (tid < 16)
+        hs.cnt[tid] = tid == 0   ? nt
+                      : tid == 1 ? binned
+                                 : __hip_atomic_load(const_cast<int32_t *>(counters) + tid,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct StoreTaskBase {
     Task *t;
     __device__ void operator()(int i, int64_t v) const { t[i].trace_base = v; }
@@ -1658,6 +1795,15 @@ hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t 
                            order, first, count, moves, move_words);
     hipLaunchKernelGGL(sw_expand_kernel, dim3((max_pairs + 15) / 16), dim3(256), 0, st, ba,
                        order, first, count, moves, move_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_binning_tiny(const BinArgs &bin, const TaskArgs &ta, int32_t *order,
+                               Task *tasks, int64_t *task_words, int64_t *total_words,
+                               const int32_t *counters, const TinyStage &hs, hipStream_t st) {
+    if (bin.npairs > kBinTinyPairs) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(sw_bin_tiny, dim3(1), dim3(1024), 0, st, bin, ta, order, tasks, task_words,
+                       total_words, counters, hs);
     return hipGetLastError();
 }
 

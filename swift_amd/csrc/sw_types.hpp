@@ -153,6 +153,18 @@ constexpr int kWaveWalkPairs = 4096;
 hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t *first,
                        const int32_t *count, int max_pairs, uint32_t *moves, int move_words,
                        hipStream_t st);
+// pass-1 binning in one workgroup for batches up to kBinTinyPairs pairs,
+// the sizes written into the host's mapped staging block (TinyStage)
+constexpr int kBinTinyPairs = 4096;
+struct TinyStage {
+    int64_t *words;   // total trace words
+    int64_t *steps;   // 2 * kRBins: sweep steps per segment
+    int32_t *seg;     // 2 * kRBins + 1: first task of each segment
+    int32_t *cnt;     // 16: the counters (ntasks, binned pairs, error flag, ...)
+};
+hipError_t launch_binning_tiny(const BinArgs &bin, const TaskArgs &ta, int32_t *order,
+                               Task *tasks, int64_t *task_words, int64_t *total_words,
+                               const int32_t *counters, const TinyStage &hs, hipStream_t st);
 // scan_sums: room for max_tasks / 2048 + 1 int64 values; bin.bin_count
 // must be zero on entry
 hipError_t launch_binning(const BinArgs &bin, const TaskArgs &ta, int32_t *order, Task *tasks,
