@@ -895,39 +895,75 @@ __device__ __forceinline__ WalkOut walk_wave(const uint4 *tr, int R, int W, int 
 #pragma unroll
     for (int m = 0; m < NMV; ++m) mvr[m] = 0;
     uint32_t mword = 0;
+    auto flush = [&](int word_index, uint32_t value) {
+#pragma unroll
+        for (int m = 0; m < NMV; ++m)
+            if (word_index == m * 64 + lane) mvr[m] = value;
+    };
     int ci = row, cj = col;
     for (;;) {
-        const int g = (int)(((uint32_t)(ci - 1) * inv) >> 16), k = (ci - 1) - g * R;
-        const int t = (cj - 1) + g, tb = t >> 2, w = k >> kl;
+        const int g = (int)(((uint32_t)(ci - 1) * inv) >> 16);
+        const int tb = ((cj - 1) + g) >> 2;
         if (g0 < 0 || g0 - g >= NG || tb0 - tb >= NT) {
             g0 = g;
             tb0 = tb;
             v = tr[(max(tb0 - dtb, 0) * W + lw) * 64 + max(g0 - dg, 0)];
         }
-        const int L = (g0 - g) * WT + w * NT + (tb0 - tb);
-        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)v.x, L);
-        const uint32_t y = (uint32_t)__builtin_amdgcn_readlane((int)v.y, L);
-        const uint32_t z = (uint32_t)__builtin_amdgcn_readlane((int)v.z, L);
-        const uint32_t u = (uint32_t)__builtin_amdgcn_readlane((int)v.w, L);
-        const uint32_t word = (t & 2) ? ((t & 1) ? u : z) : ((t & 1) ? y : x);
-        const int c = (int)((word >> (bitb + 2 * (k & kmask))) & 3u);
-        const int ni = (c & 2) ? ci - 1 : ci;             // diag (3), up (2)
-        const int nj = (c == 3 || c == 1) ? cj - 1 : cj;  // diag (3), left (1)
-        if (ni == ci && nj == cj) break;                  // a start pointer
-        const int op = nj == cj ? 1 : ni == ci ? 2 : 0;
-        if (op != 1) ++nrb;
+        // Lane l reads the code of cell (ci - l, cj - l), l steps down the
+        // diagonal, from the window lane holding its granule; the walk then
+        // takes every leading diagonal move at once.
+        const int il = ci - lane, jl = cj - lane;
+        const int gl = (int)(__umul24((uint32_t)max(il - 1, 0), inv) >> 16);
+        const int kl_ = (il - 1) - gl * R, tl = (jl - 1) + gl;
+        const bool in_win = il >= 1 && jl >= 1 && g0 - gl < NG && tb0 - (tl >> 2) < NT;
+        const int L = in_win ? (g0 - gl) * WT + (kl_ >> kl) * NT + (tb0 - (tl >> 2)) : 0;
+        const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute(L << 2, (int)v.x);
+        const uint32_t y = (uint32_t)__builtin_amdgcn_ds_bpermute(L << 2, (int)v.y);
+        const uint32_t z = (uint32_t)__builtin_amdgcn_ds_bpermute(L << 2, (int)v.z);
+        const uint32_t u = (uint32_t)__builtin_amdgcn_ds_bpermute(L << 2, (int)v.w);
+        const uint32_t word = (tl & 2) ? ((tl & 1) ? u : z) : ((tl & 1) ? y : x);
+        const int cl = in_win ? (int)((word >> (bitb + 2 * (kl_ & kmask))) & 3u) : -1;
+        const uint64_t stop = __ballot(cl != 3);
+        // the first cell off the diagonal run (or outside the window)
+        const int first = stop ? (int)__builtin_ctzll(stop) : 64;
+        if (first > 0) {
+            // `first` diagonal moves (op 0: no bits to set); the move words
+            // they complete are flushed as the count passes them
+            nrb += first;
+            start = cj - first + 1;
+            ci -= first;
+            cj -= first;
+            int rest = first;
+            while ((tick & 15) + rest >= 16) {
+                const int adv = 16 - (tick & 15);
+                flush(tick >> 4, mword);
+                mword = 0;
+                tick += adv;
+                rest -= adv;
+            }
+            tick += rest;
+            if (ci == 0 || cj == 0) break;  // the target is in row/column 0
+        }
+        if (first >= 64) continue;
+        // that cell is now the current one; outside the window, a window is
+        // loaded around it first
+        if (!__builtin_amdgcn_readlane((int)in_win, first)) continue;
+        const int c = __builtin_amdgcn_readlane(cl, first);
+        if (c == 0) break;  // a start pointer: stop without emitting
+        // up (2) or left (1)
+        const int di = c >> 1, dj = c & 1;
+        const int op = dj == 0 ? 1 : 2;
+        nrb += dj;
         mword |= (uint32_t)op << (2 * (tick & 15));
         if ((tick & 15) == 15) {
-#pragma unroll
-            for (int m = 0; m < NMV; ++m)
-                if ((tick >> 4) == m * 64 + lane) mvr[m] = mword;
+            flush(tick >> 4, mword);
             mword = 0;
         }
         start = cj;
         ++tick;
-        if (ni == 0 || nj == 0) break;  // the target is in row/column 0
-        ci = ni;
-        cj = nj;
+        ci -= di;
+        cj -= dj;
+        if (ci == 0 || cj == 0) break;  // the target is in row/column 0
     }
     if (tick & 15) {
 #pragma unroll
