@@ -149,7 +149,7 @@ hipError_t launch_batch_wide(const BatchArgs &ba, int cu_count, int max_tasks, h
 // be null: 0; max_pairs bounds *count); moves holds move_words words per pair
 // id, move_words >= (kMaxBatchQ + kMaxBatchR) / 16 + 1.  Under
 // kWaveWalkPairs pairs a wave walks each pair, else a lane.
-constexpr int kWaveWalkPairs = 4096;
+constexpr int kWaveWalkPairs = 16384;
 hipError_t launch_walk(const BatchArgs &ba, const int32_t *order, const int32_t *first,
                        const int32_t *count, int max_pairs, uint32_t *moves, int move_words,
                        hipStream_t st);
