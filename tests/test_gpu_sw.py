@@ -89,6 +89,36 @@ def test_plain_shapes_vs_oracle(A, orc, ql, rl):
         compare(res.record(i), orc.sw(rs[i], qs[i], ref_len=rl), (ql, rl, i))
 
 
+def test_plain_long_paths_vs_oracle(A, orc):
+    """Queries of 1,000 bases with indels against 2,000-base references:
+    paths past 1,024 moves, so the wave-per-pair walk keeps its moves in more
+    than one register per lane (64 words each)."""
+    rnd = random.Random(71)
+    ql, rl = 1000, 2000
+    qs, rs = [], []
+    for _ in range(48):
+        reg = bytes(rnd.choice(b"ACGT") for _ in range(rl))
+        o = rnd.randint(0, 400)
+        src = bytearray(reg[o:o + 1400])
+        q = bytearray()
+        i = 0
+        while len(q) < ql and i < len(src):
+            r = rnd.random()
+            if r < 0.05:
+                i += rnd.randint(1, 3)  # deletion from the query
+            elif r < 0.10:
+                q += bytes(rnd.choice(b"ACGT") for _ in range(rnd.randint(1, 3)))
+            else:
+                q.append(src[i] if rnd.random() > 0.03 else rnd.choice(b"ACGT"))
+                i += 1
+        qs.append(bytes(q[:ql].ljust(ql, b"A")))
+        rs.append(reg)
+    res = A.align_sequences(qs, rs, ql, rl)
+    assert max(res.length) > 1024
+    for i in range(len(qs)):
+        compare(res.record(i), orc.sw(rs[i], qs[i], ref_len=rl), (ql, rl, i))
+
+
 def test_plain_chunks_vs_oracle(A, orc):
     """A call of many chunks (7 pairs each) through the one scratch buffer;
     every pair against the oracle, twice in a row (the second call with
