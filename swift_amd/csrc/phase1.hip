@@ -1588,6 +1588,39 @@ hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildS
     return hipGetLastError();
 }
 
+// Sets *flag when any byte of seq[0, n) is outside ACGTNacgtn (b | 0x20 is
+// one of acgtn exactly for those ten bytes); 16 bytes a thread per step.
+__global__ void __launch_bounds__(256) seq_plain_scan(const char *seq, int64_t n, int32_t *flag) {
+    const int64_t n16 = n >> 4;
+    const uint4 *v = reinterpret_cast<const uint4 *>(seq);
+    auto bad = [](uint32_t w) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t c = ((w >> (8 * b)) & 0xFFu) | 0x20u;
+            any |= (c != 'a' && c != 'c' && c != 'g' && c != 't' && c != 'n') ? 1u : 0u;
+        }
+        return any;
+    };
+    uint32_t found = 0;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) {
+        const uint4 x = v[i];
+        found |= bad(x.x) | bad(x.y) | bad(x.z) | bad(x.w);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 15)) {
+        const uint32_t c = (uint32_t)(unsigned char)seq[(n16 << 4) + threadIdx.x] | 0x20u;
+        found |= (c != 'a' && c != 'c' && c != 'g' && c != 't' && c != 'n') ? 1u : 0u;
+    }
+    if (found) atomicOr(flag, 1);
+}
+
+hipError_t launch_seq_plain_scan(const char *seq, int64_t n, int32_t *flag, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(((n >> 4) + 255) / 256 + 1, 4096);
+    hipLaunchKernelGGL(seq_plain_scan, dim3((unsigned)blocks), dim3(256), 0, st, seq, n, flag);
+    return hipGetLastError();
+}
+
 hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks) {
     if (a.rd.nreads <= 0) return hipSuccess;
     const bool hash_first = a.hash_first && mid_blocks > 0 && a.ix.cid_ok;

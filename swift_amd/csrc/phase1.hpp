@@ -55,6 +55,9 @@ struct IndexDev {
     // cluster id is this minus its query offset plus kShiftBias, so the hash
     // tier reads 4 bytes per hit instead of the 8-byte entry
     const uint32_t *gpos;
+    // 1: every base of seq is one of ACGTNacgtn (so no window can send a pair
+    // to the int32 re-run for its reference bytes)
+    int plain_bases;
 };
 
 struct IndexBuildScratch {
@@ -130,6 +133,8 @@ hipError_t launch_index_fill(const IndexDev &ix, int64_t ntup, const IndexBuildS
 // mid_blocks: persistent workgroups of map_strands_mid (0 when no strand can
 // pass kMapCap hits)
 hipError_t launch_map(const MapArgs &a, hipStream_t st, int mid_blocks);
+// *flag |= 1 when seq[0, n) holds a byte outside ACGTNacgtn
+hipError_t launch_seq_plain_scan(const char *seq, int64_t n, int32_t *flag, hipStream_t st);
 hipError_t launch_records(const MapArgs &m, const PairsDev &p, int before, int win,
                           hipStream_t st);
 hipError_t launch_choose_best(const PairsDev &p, int nreads, const int32_t *score,

@@ -166,6 +166,17 @@ static int index_build(const char *bases, const int64_t *off, const int64_t *len
             SWIFT_HIP_TRY(hipMemcpy(ix->seq.as<char>() + ix->h_off[s], bases + off[s], (size_t)len[s],
                                     hipMemcpyHostToDevice));
     {
+        // ACGTN bases only? (then no alignment window can need the int32
+        // re-run for its reference bytes)
+        DevArr flag;
+        if ((rc = flag.alloc(64))) return rc;
+        SWIFT_HIP_TRY(hipMemset(flag.ptr, 0, 4));
+        SWIFT_HIP_TRY(launch_seq_plain_scan(ix->seq.as<char>(), nb, flag.as<int32_t>(), nullptr));
+        int32_t any = 0;
+        SWIFT_HIP_TRY(hipMemcpy(&any, flag.ptr, 4, hipMemcpyDeviceToHost));
+        ix->plain_bases = any ? 0 : 1;
+    }
+    {
         // cluster-id bases: cid = cid0[s] + shift + kShiftBias stays below
         // 2^32 - 1 (the empty-slot marker of the hash tier)
         std::vector<uint32_t> c0(std::max(nseq, 1), 0);
@@ -480,6 +491,7 @@ int align_enqueue(const MapArgs &m, const PairsDev &p, Workspace *sws, int nread
     io.qcons = rec.qry_cons;
     io.rcons = rec.ref_cons;
     io.cons_off = p.coff;
+    io.refs_plain = m.ix.plain_bases;
     if (sync)
         rc = sws ? sw_batch_run_in(sws, io, sc, st) : sw_batch_run(io, sc, st, true);
     else

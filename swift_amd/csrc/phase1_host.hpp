@@ -39,6 +39,7 @@ struct IndexObj {
     int64_t nhash = 0, ntup = 0, total = 0;
     DevArr seq, seq_off, seq_len, tup0, count, start, actual, cursor, bucket, entry, cid0, gpos;
     int cid_ok = 0;
+    int plain_bases = 0;  // every base one of ACGTNacgtn
     std::vector<int64_t> h_off, h_len;
     std::vector<std::string> names;
 
@@ -60,6 +61,7 @@ struct IndexObj {
         d.entry = entry.as<uint64_t>();
         d.cid0 = cid0.as<const uint32_t>();
         d.cid_ok = cid_ok;
+        d.plain_bases = plain_bases;
         d.gpos = gpos.as<const uint32_t>();
         return d;
     }

@@ -676,7 +676,14 @@ static int batch_run(Workspace &w, const BatchIO &io, const Scoring &sc, hipStre
     // ---- pass 2: int32 re-run of pairs flagged by the int16 kernel
     int64_t words2 = words1;
     int32_t nflagged = 1;
-    if (!force32 && sync_sizes) {
+    // no pair can be flagged: every reference byte is ACGTN and no best score
+    // reaches the int16 limit (a score is at most maxq matches); then the
+    // flagged-count round trip is skipped
+    const bool no_rerun =
+        io.refs_plain && (int64_t)io.maxq * std::max(sc.match, 0) * 4 < (int64_t)h_limit;
+    if (!force32 && no_rerun) {
+        nflagged = 0;
+    } else if (!force32 && sync_sizes) {
         // the int16 kernels count what they flag: with none, pass 2 is skipped
         if ((rc = gather({{cnt + kCntFlagged, 1}}, {(int)offsetof(HS, cnt)}))) return rc;
         nflagged = w.stage->cnt[0];

@@ -68,6 +68,10 @@ struct BatchIO {
     int32_t *score, *start, *end, *alen;
     char *qcons, *rcons;
     const int64_t *cons_off;
+    // 1: every reference byte is one of ACGTNacgtn (the caller knows it, as
+    // the pipeline does from its index); with scores that cannot reach the
+    // int16 limit either, no pair can take the int32 re-run
+    int refs_plain = 0;
 };
 
 int validate_scoring(const Scoring &s, int maxq, int maxr, bool *force32);
