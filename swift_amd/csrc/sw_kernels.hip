@@ -860,17 +860,19 @@ __global__ void __launch_bounds__(256) sw_walk_kernel(BatchArgs a, const int32_t
     }
 }
 
-// Deferred walk of one pair by a whole wave, for batches too small to fill
+// Deferred walk of one pair by a whole wave, for calls too small to fill
 // the chip (a lane per pair leaves each wave waiting on one dependent
 // granule load after another, and every lane's wait is the wave's).  The
-// walk itself is wave-uniform and runs in scalar registers; the trace
-// around it is loaded as a window of granules, one per lane -- lane groups
-// (g0 - NG, g0], every trace word, step blocks (tb0 - NT, tb0] -- and each
-// cell's code read from its lane with v_readlane.  The walk only moves up
-// and left, so it stays in a window until its lane group or step block
-// passes the window's lower edge; a new window is then loaded at the
-// current cell, one load per lane.  Same path, moves and results as
-// walk_cells.  tr: the pair's lane group at step 0, word 0.
+// walk's state is wave-uniform; the trace around it is loaded as a window
+// of granules, one per lane -- lane groups (g0 - NG, g0], every trace word,
+// step blocks (tb0 - NT, tb0].  Lane l reads the code of the cell l steps
+// down the diagonal from the window lane holding its granule (ds_bpermute);
+// a ballot gives the length of the leading diagonal run, which the walk
+// takes at once, then the move off it.  The walk only moves up and left, so
+// it stays in a window until its lane group or step block passes the
+// window's lower edge; a new window is then loaded at the current cell, one
+// load per lane.  Same path, moves and results as walk_cells.  tr: the
+// pair's lane group at step 0, word 0.
 template <int NMV>
 __device__ __forceinline__ WalkOut walk_wave(const uint4 *tr, int R, int W, int kl, int bitb,
                                              int score, int row, int col, int qlen, int rlen,
