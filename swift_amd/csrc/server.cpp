@@ -46,7 +46,24 @@ struct SrvRequest {
     std::vector<int64_t> rdist;
     std::vector<char> qcons, rcons;  // packed: record k at cpos[k], alen bytes
     std::vector<int64_t> cpos;
+    // A large request is served as parts: consecutive ranges of its reads,
+    // queued as requests of their own so that idle slots take them at once.
+    // A part knows its parent and its first read; the parent is done when its
+    // last part is, and its records are its parts' in order.
+    SrvRequest *parent = nullptr;
+    int read0 = 0;
+    std::vector<SrvRequest *> parts;
+    int parts_left = 0;
+    SrvRequest() = default;
+    SrvRequest(const SrvRequest &) = delete;
+    SrvRequest &operator=(const SrvRequest &) = delete;
+    ~SrvRequest() {
+        for (SrvRequest *q : parts) delete q;
+    }
 };
+
+// requests above this many reads are split into parts over the slots
+constexpr int kSplitReads = 1024;
 
 // Pinned host staging (hipHostMalloc), grown on demand.
 struct PinBuf {
@@ -266,8 +283,24 @@ static void slot_main(Server *S, SrvSlot *sl) {
                 r->done = true;
                 r->t_done = now;
                 nr += (int)r->len.size();
+                SrvRequest *P = r->parent;
+                if (!P) {
+                    S->completed += 1;
+                    continue;
+                }
+                if (rc && !P->status) {
+                    P->status = rc;
+                    P->err = err;
+                }
+                if (--P->parts_left == 0) {
+                    int n = 0;
+                    for (SrvRequest *q : P->parts) n += q->nrec;
+                    P->nrec = n;
+                    P->t_done = now;
+                    P->done = true;
+                    S->completed += 1;
+                }
             }
-            S->completed += (int64_t)batch.size();
             S->batches += 1;
             S->batched_reads += nr;
         }
@@ -375,23 +408,56 @@ int swift_server_submit(swift_server *h, const char *reads, const int64_t *off, 
         delete r;
         return kErrCapacity;
     }
-    r->bases.resize((size_t)nb);
-    r->off.resize(nreads);
-    r->len.assign(len, len + nreads);
-    int64_t b = 0;
-    for (int i = 0; i < nreads; ++i) {
-        memcpy(r->bases.data() + b, reads + off[i], (size_t)len[i]);
-        r->off[i] = b;
-        b += len[i];
+    // reads [a, b) of the call into request q
+    auto fill = [&](SrvRequest *q, int a, int b) {
+        int64_t nbq = 0;
+        for (int i = a; i < b; ++i) nbq += len[i];
+        q->bases.resize((size_t)nbq);
+        q->off.resize(b - a);
+        q->len.assign(len + a, len + b);
+        int64_t o = 0;
+        for (int i = a; i < b; ++i) {
+            memcpy(q->bases.data() + o, reads + off[i], (size_t)len[i]);
+            q->off[i - a] = o;
+            q->max_len = std::max(q->max_len, len[i]);
+            o += len[i];
+        }
+    };
+    const int nparts =
+        S.nslots > 1 && nreads > kSplitReads
+            ? std::min(S.nslots, (nreads + kSplitReads - 1) / kSplitReads)
+            : 1;
+    if (nparts == 1) {
+        r->max_len = 0;
+        fill(r, 0, nreads);
+    } else {
+        for (int k = 0; k < nparts; ++k) {
+            auto *q = new SrvRequest;
+            const int a = (int)((int64_t)nreads * k / nparts);
+            const int b = (int)((int64_t)nreads * (k + 1) / nparts);
+            fill(q, a, b);
+            q->parent = r;
+            q->read0 = a;
+            r->parts.push_back(q);
+        }
+        r->parts_left = nparts;
     }
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(S.mu);
         r->id = S.next_id++;
         S.reqs[r->id] = r;
-        S.queue.push_back(r);
+        if (nparts == 1) {
+            S.queue.push_back(r);
+        } else {
+            for (SrvRequest *q : r->parts) {
+                q->t_submit = r->t_submit;
+                S.queue.push_back(q);
+            }
+        }
     }
-    S.cv_work.notify_one();
+    if (nparts == 1) S.cv_work.notify_one();
+    else S.cv_work.notify_all();
     *id = r->id;
     return kOk;
 }
@@ -441,23 +507,35 @@ int swift_server_fetch(swift_server *h, uint64_t id, const swift_records *out) {
         S.reqs.erase(it);
     }
     if (out) {
-        const int n = r->nrec;
         int32_t *dst[kRecI32] = {out->read, out->strand, out->ref_idx, out->shift, out->ref_pos,
                                  out->nbiggest, out->score, out->start, out->end, out->alen,
                                  out->keep, out->ntied};
-        for (int c = 0; c < kRecI32; ++c)
-            if (dst[c] && n) memcpy(dst[c], r->col[c].data(), 4 * (size_t)n);
-        if (out->ref_distance && n) memcpy(out->ref_distance, r->rdist.data(), 8 * (size_t)n);
-        for (int i = 0; i < n; ++i) {
-            const int64_t L = std::min<int64_t>(r->col[9][i], out->cons_stride);
-            if (out->qry_cons)
-                memcpy(out->qry_cons + (int64_t)i * out->cons_stride, r->qcons.data() + r->cpos[i],
-                       (size_t)L);
-            if (out->ref_cons)
-                memcpy(out->ref_cons + (int64_t)i * out->cons_stride, r->rcons.data() + r->cpos[i],
-                       (size_t)L);
-        }
-        if (out->nrecords) *out->nrecords = n;
+        // the request's records, or its parts' one after another (their read
+        // indices shifted by each part's first read)
+        int at = 0;
+        auto put = [&](const SrvRequest *q, int read0) {
+            const int n = q->nrec;
+            for (int c = 0; c < kRecI32; ++c)
+                if (dst[c] && n) memcpy(dst[c] + at, q->col[c].data(), 4 * (size_t)n);
+            if (read0 && dst[0])
+                for (int i = 0; i < n; ++i) dst[0][at + i] += read0;
+            if (out->ref_distance && n)
+                memcpy(out->ref_distance + at, q->rdist.data(), 8 * (size_t)n);
+            for (int i = 0; i < n; ++i) {
+                const int64_t L = std::min<int64_t>(q->col[9][i], out->cons_stride);
+                if (out->qry_cons)
+                    memcpy(out->qry_cons + (int64_t)(at + i) * out->cons_stride,
+                           q->qcons.data() + q->cpos[i], (size_t)L);
+                if (out->ref_cons)
+                    memcpy(out->ref_cons + (int64_t)(at + i) * out->cons_stride,
+                           q->rcons.data() + q->cpos[i], (size_t)L);
+            }
+            at += n;
+        };
+        if (r->parts.empty()) put(r, 0);
+        else
+            for (const SrvRequest *q : r->parts) put(q, q->read0);
+        if (out->nrecords) *out->nrecords = at;
     }
     delete r;
     return kOk;

@@ -62,6 +62,32 @@ def test_results_same_alone_and_in_mix(world, orc):
         assert_records_match(g, orc.pipeline_mt(oix, r, 8), i)
 
 
+def test_large_requests_served_in_parts(world, orc):
+    """Requests above 1,024 reads are served as parts over the slots; each
+    one's records are still those of the request alone, in read order, and
+    the oracle's."""
+    from swift_amd import synth
+    from swift_amd.serve import Server
+    ix, _, seqs = world
+    rng = np.random.default_rng(77)
+    reqs = []
+    for n in (1025, 2500, 4000, 700):
+        reads, _ = synth.sample_reads(rng, [np.frombuffer(s, np.uint8) for s in seqs], n,
+                                      int(rng.integers(80, 301)), 0.015, 0.0015)
+        reqs.append(reads)
+    alone = [ix.align_reads(r, 3) for r in reqs]
+    with Server(ix, nslots=4, mem_budget=16 << 30, max_batch_reads=4096) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        got = [srv.result(i)[0] for i in ids]
+        st = srv.stats()
+    assert st["completed"] == len(reqs)
+    for i, (g, a) in enumerate(zip(got, alone)):
+        same(g, a, i)
+    oix = orc.Index(seqs, 12, 1000)
+    for i, (g, r) in enumerate(zip(got, reqs)):
+        assert_records_match(g, orc.pipeline_mt(oix, r, 8), i)
+
+
 def test_served_int32_reruns_vs_oracle(orc):
     """References holding bytes outside ACGTNacgtn (IUPAC R and Y, about 1 %
     of the bases): their windows are re-run with byte semantics in the
