@@ -421,7 +421,10 @@ int swift_server_create(const swift_index *ix, int nslots, int64_t mem_budget_by
                         int max_batch_reads, int max_read_len, int max_hits, int match,
                         int mismatch, int gap_open, int gap_ext, swift_server **out);
 /* Copies the reads (host memory) and queues them; *id names the request.
- * Requests larger than one slot are rejected (SWIFT_ERR_CAPACITY). */
+ * Requests larger than one slot are rejected (SWIFT_ERR_CAPACITY).  With
+ * more than one slot, a request of more than 1024 reads is queued as up to
+ * one part per slot (consecutive ranges of its reads); its records still
+ * come back together, in read order. */
 int swift_server_submit(swift_server *s, const char *reads, const int64_t *off, const int32_t *len,
                         int nreads, uint64_t *id);
 /* Waits up to timeout_ms (< 0: no limit); *nrecords and the submit-to-done
