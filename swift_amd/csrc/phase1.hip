@@ -1313,8 +1313,8 @@ map_strands_filter(MapArgs a, const int32_t *src, const int32_t *src_count, int3
         // clear the bitmaps and, when it was used, the table
 #pragma unroll
         for (int i = t; i < 2 * kWords; i += THREADS) bm[i] = 0;
-    if constexpr (kOwnSep)
-        for (int i = t; i < CAP / 2; i += THREADS) own_sep[i] = 0;
+        if constexpr (kOwnSep)
+            for (int i = t; i < CAP / 2; i += THREADS) own_sep[i] = 0;
         if (ncand_all > 0 && ncand_all <= SMALLCAP) {
             for (int i = t; i < SMALL; i += THREADS) {
                 skey[i] = kEmpty;

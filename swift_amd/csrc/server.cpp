@@ -392,7 +392,6 @@ int swift_server_submit(swift_server *h, const char *reads, const int64_t *off, 
     }
     Server &S = h->s;
     auto *r = new SrvRequest;
-    int64_t nb = 0;
     for (int i = 0; i < nreads; ++i) {
         if (len[i] < 0 || len[i] > S.max_read_len) {
             set_error("read %d has length %d (server limit %d)", i, len[i], S.max_read_len);
@@ -400,7 +399,6 @@ int swift_server_submit(swift_server *h, const char *reads, const int64_t *off, 
             return kErrCapacity;
         }
         r->max_len = std::max(r->max_len, len[i]);
-        nb += len[i];
     }
     if (nreads > S.max_batch_reads || slot_bytes(nreads, r->max_len, S.max_hits) > S.slot_cap) {
         set_error("request of %d reads exceeds one slot (%d reads, %lld B)", nreads,
